@@ -1,0 +1,262 @@
+#!/usr/bin/env python3
+"""Benchmark of the libspmv hot path on one MI355X.
+
+Step (the JSON `value`): one pass of the reference's matvec loop
+(demos/matvecmain.cpp:70-86) on one rank's share of a 27-point stencil:
+160^3 owned rows plus one 160x160 ghost plane fed through L2GMap from a
+seeded host halo buffer (FP64 CSR, 109,443,836 stored entries):
+    L2GMap::update(x)   device pack + halo unpack
+    y = A * x           ordered FP64 CSR SpMV (bit-identical to the reference)
+value = algorithmic SpMV bytes per step / measured step time (GB/s), with
+    B = nnz*(8+4) + 4*(M+1) + 8*ncols + 8*M   (x read once, y written once).
+
+Also reported: the SpMV kernel's roofline (HIP events on its stream), the
+CPU oracle timed on the host (1 thread, bounded sample), and the CG
+time-to-1e-10 of FP64 CG versus mixed-precision refinement (FP32 and BF16
+values) on the 256^3 7-point Poisson system.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SpMV GB/s (% HBM peak); CG time-to-1e-10 rel. residual, mixed vs FP64"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(nnz, nrows, ncols, val_bytes=8, vec_bytes=8):
+    return nnz * (val_bytes + 4) + 4 * (nrows + 1) + vec_bytes * ncols + vec_bytes * nrows
+
+
+def timed_loop(torch, fn, iters):
+    """Average ms of fn() over iters, HIP events on the current stream."""
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(iters):
+        fn()
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / iters
+
+
+def run_cg_section(torch, S, P, O, n, seed, cpu_iters):
+    """FP64 CG vs mixed-precision refinement to 1e-10 on the n^3 7-point
+    Poisson system (b uniform[-1,1])."""
+    dev = "cuda:0"
+    t0 = time.time()
+    A = P.stencil(7, n, n, n)
+    log(f"[cg] generated 7-point {n}^3: {A.nnz} nnz in {time.time() - t0:.1f}s")
+    M64 = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    M32 = M64.like("f32")
+    M16 = M64.like("bf16")
+    b = torch.from_numpy(P.uniform(seed, A.nrows, -1, 1)).to(dev)
+    bnorm = torch.linalg.norm(b).item()
+
+    def true_rel(x):
+        r = M64.mult(x)
+        return torch.linalg.norm(b - r).item() / bnorm
+
+    out = {"workload": f"7-point Poisson {n}^3, FP64 CSR ({A.nnz} nnz), b uniform[-1,1] seed {seed}",
+           "rtol": 1e-10}
+    # warm up the workspaces once (not timed)
+    S.cg(M64, b, 2, 1e-10)
+    S.ir_cg(M64, M32, b, rtol=1e-2, inner_rtol=1e-1, max_outer=1)
+    S.ir_cg(M64, M16, b, rtol=1e-2, inner_rtol=1e-1, max_outer=1)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    x, k = S.cg(M64, b, 100000, 1e-10)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t
+    out["fp64"] = {"seconds": t64, "iterations": k, "true_rel_residual": true_rel(x),
+                   "ms_per_iteration": 1e3 * t64 / max(k, 1)}
+    log(f"[cg] fp64: {t64:.3f}s {k} its")
+    for name, Ml in (("fp32_ir", M32), ("bf16_ir", M16)):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x, st = S.ir_cg(M64, Ml, b, rtol=1e-10, inner_rtol=1e-4)
+        torch.cuda.synchronize()
+        tm = time.perf_counter() - t
+        out[name] = {"seconds": tm, "outer": st["outer_its"], "inner_iterations": st["inner_its"],
+                     "inner_per_outer": st["inner_hist"], "true_rel_residual": true_rel(x),
+                     "speedup_vs_fp64": t64 / tm}
+        log(f"[cg] {name}: {tm:.3f}s outer {st['outer_its']} inner {st['inner_its']}")
+    if cpu_iters > 0:
+        oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
+        bh = b.cpu().numpy()
+        t = time.perf_counter()
+        O.cg(oc, bh, cpu_iters, 1e-30)
+        tc = time.perf_counter() - t
+        out["cpu_fp64_ms_per_iteration"] = 1e3 * tc / cpu_iters
+        out["cpu_fp64_estimated_seconds"] = tc / cpu_iters * k
+        out["cpu_sample"] = f"{cpu_iters} oracle CG iterations, 1 thread"
+    del M16, M32, M64
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=160, help="owned grid edge of the 27-point share")
+    ap.add_argument("--cg-n", type=int, default=256, help="grid edge of the CG comparison")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--mode", choices=["bench", "spmv", "copy"], default="bench",
+                    help="spmv: only the SpMV kernel loop; copy: the calibration copy kernel")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the multi-GPU exchange is not part of this build "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import spmv_amd as S
+    from spmv_amd import problems as P
+
+    dev = "cuda:0"
+    torch.cuda.set_device(0)
+
+    if args.mode == "copy":
+        n = 1 << 28  # 2 GiB of doubles
+        src = torch.rand(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(args.warmup):
+            S.copy_f64(dst, src)
+        ms = timed_loop(torch, lambda: S.copy_f64(dst, src), args.steps)
+        print(json.dumps({"mode": "copy", "bytes_read": 8 * n, "bytes_written": 8 * n,
+                          "ms": ms, "GBps": 16 * n / ms / 1e6}))
+        return
+
+    n = args.n
+    t0 = time.time()
+    A = P.stencil(27, n, n, n, ghost_planes=1)
+    log(f"[spmv] generated 27-point {n}^3 + ghost plane: {A.nnz} nnz in {time.time() - t0:.1f}s")
+    l2g = S.L2GMap(**P.slab_decomposition(n, n, n))
+    halo = P.uniform(2, l2g.num_ghosts(), -1, 1)
+    l2g.set_halo(halo)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, col_map=l2g, row_map=S.L2GMap(A.nrows))
+    x = torch.from_numpy(P.uniform(1, A.ncols, -1, 1)).to(dev)
+    y = torch.empty(A.nrows, dtype=torch.float64, device=dev)
+    B = spmv_bytes(A.nnz, A.nrows, A.ncols)
+    log(f"[spmv] matrix on device: {M.nblocks} row blocks, {B} algorithmic bytes per SpMV")
+
+    def step():
+        l2g.update(x)
+        M.mult(x, out=y)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    if args.mode == "spmv":
+        ms = timed_loop(torch, lambda: M.mult(x, out=y), args.steps)
+        print(json.dumps({"mode": "spmv", "ms": ms, "GBps": B / ms / 1e6}))
+        return
+
+    # ---- timed steps (the JSON value)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    step_ms_ev = timed_loop(torch, step, args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    ms_per_step = 1e3 * wall / args.steps
+    value = B / (ms_per_step * 1e-3) / 1e9
+    # ---- SpMV kernel alone, HIP events on the launch stream
+    spmv_ms = timed_loop(torch, lambda: M.mult(x, out=y), max(args.steps, 20))
+    achieved = B / (spmv_ms * 1e-3) / 1e9
+    log(f"[spmv] step {ms_per_step:.3f} ms (events {step_ms_ev:.3f}), SpMV kernel {spmv_ms:.4f} ms "
+        f"= {achieved:.0f} GB/s")
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "spmv_27pt_160_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"27-point stencil (diag 26, off -1) on a {n}^3 owned block plus one "
+                         f"{n}x{n} ghost plane fed through L2GMap from a seeded host halo; "
+                         "step = L2GMap::update + FP64 CSR SpMV"),
+            "rows": A.nrows, "cols": A.ncols, "nnz": A.nnz,
+            "algorithmic_bytes_per_spmv": B,
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel_ms": round(spmv_ms, 5)},
+    }
+
+    if not args.no_cpu:
+        import oracle as O
+        oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
+        xh = x.cpu().numpy()
+        reps, tc = 0, 0.0
+        while tc < args.cpu_seconds or reps < 2:
+            t = time.perf_counter()
+            yc = O.spmv(oc, xh)
+            tc += time.perf_counter() - t
+            reps += 1
+        cpu_gbs = B * reps / tc / 1e9
+        same = bool(np.array_equal(yc, y.cpu().numpy()))
+        ncores = os.cpu_count() or 1
+        t = time.perf_counter()
+        for _ in range(3):
+            O.spmv(oc, xh, threads=ncores)
+        tmc = (time.perf_counter() - t) / 3
+        result["cpu_baseline"] = {"value": round(cpu_gbs, 3), "unit": "GB/s", "cores": 1,
+                                  "kind": "port",
+                                  "sample": f"{reps} oracle SpMVs of the same matrix and x "
+                                            f"({tc:.1f}s), 1 thread; results bit-identical to "
+                                            f"the GPU: {same}",
+                                  "all_cores": {"cores": ncores,
+                                                "value": round(B / tmc / 1e9, 3)}}
+        log(f"[cpu] {cpu_gbs:.2f} GB/s (1 thread), {B / tmc / 1e9:.2f} GB/s ({ncores} threads)")
+
+    # CG on the same share, zero halo (homogeneous neighbour), to 1e-10
+    l2g.set_halo(None)
+    b = torch.from_numpy(P.uniform(3, A.nrows, -1, 1)).to(dev)
+    S.cg(M, b, 2, 1e-10)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    xs, k = S.cg(M, b, 100000, 1e-10)
+    torch.cuda.synchronize()
+    tcg = time.perf_counter() - t
+    result["cg_27pt"] = {"seconds": round(tcg, 4), "iterations": k, "rtol": 1e-10,
+                         "halo": "zero (homogeneous neighbour values for the direction vector)"}
+    log(f"[cg27] {tcg:.3f}s {k} its")
+    del M, xs
+
+    if not args.no_cg:
+        import oracle as O
+        result["cg"] = run_cg_section(torch, S, P, O, args.cg_n, 5,
+                                      0 if args.no_cpu else 3)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

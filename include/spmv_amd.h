@@ -1,0 +1,240 @@
+/*
+ * spmv_amd.h -- C ABI of the MI355X (gfx950) implementation of libspmv's
+ * hot path: CSR SpMV (Matrix<T>::operator*), the ghost update of the
+ * local-to-global map (L2GMap::update / reverse_update) and the conjugate
+ * gradient solver (spmv::cg), plus the three extensions of this library
+ * (mixed-precision refinement, batched persistent CG, adjoint / SDDMM).
+ *
+ * Conventions
+ *   - Every function returns 0 (SPMVX_OK) on success or a positive
+ *     SPMVX_ERR_* code; spmvx_last_error() then returns the message (thread
+ *     local).  Messages raised where the reference throws use the
+ *     reference's text (e.g. "Ghosts must be sorted").
+ *   - Pointers named *_dev are device pointers (hipMalloc / torch CUDA
+ *     tensors), *_host are host pointers.  Sizes are element counts.
+ *   - `stream` is a hipStream_t passed as void* (NULL = null stream).  All
+ *     work is enqueued on it; functions that return host results
+ *     synchronise it.
+ *   - Vector precision follows the matrix value type: SPMVX_F64 matrices
+ *     take double vectors; SPMVX_F32 and SPMVX_BF16 matrices take float
+ *     vectors (BF16 values are widened to FP32, FP32 accumulation).
+ *   - Handles are not thread safe; calls on one handle must be serialised.
+ */
+#ifndef SPMV_AMD_H
+#define SPMV_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPMVX_ABI_VERSION 1
+
+enum spmvx_status
+{
+  SPMVX_OK = 0,
+  SPMVX_ERR_ARG = 1,         /* invalid argument (reference: runtime_error) */
+  SPMVX_ERR_HIP = 2,         /* HIP runtime failure                         */
+  SPMVX_ERR_STATE = 3,       /* handle in the wrong state                   */
+  SPMVX_ERR_UNSUPPORTED = 4  /* not available in this build                 */
+};
+
+/* Scalar types; replaces spmv::mpi_type<T>() (spmv/mpi_type.h:11-33). */
+enum spmvx_dtype
+{
+  SPMVX_F64 = 0,
+  SPMVX_F32 = 1,
+  SPMVX_BF16 = 2 /* matrix values only */
+};
+
+typedef struct spmvx_matrix_s* spmvx_matrix;
+typedef struct spmvx_l2gmap_s* spmvx_l2gmap;
+typedef struct spmvx_batch_s* spmvx_batch;
+typedef void* spmvx_stream;
+
+/* Last error message of the calling thread ("" if none). */
+const char* spmvx_last_error(void);
+/* SPMVX_ABI_VERSION of the loaded library. */
+int spmvx_abi_version(void);
+
+/* ===================================================================== */
+/* L2GMap (spmv/L2GMap.h:29-30, spmv/L2GMap.cpp:15-122)                   */
+/* ===================================================================== */
+/*
+ * Builds the exchange plan of rank `rank` in a decomposition of `nranks`
+ * contiguous ownership ranges (local_sizes[q] indices owned by rank q, in
+ * rank order: the MPI_Allgather of L2GMap.cpp:25).  One process drives one
+ * rank: the other ranks are virtual, and their side of the symmetric
+ * neighbour discovery (L2GMap.cpp:57-115) is given explicitly:
+ *   ghosts[0..nghosts)           this rank's ghost global indices, sorted
+ *                                ascending (L2GMap.h:27-30)
+ *   remote_ptr[0..nranks], remote_ghosts[...]
+ *                                for every rank q, the sorted global indices
+ *                                of q's ghosts that this rank owns
+ *                                (remote_ptr may be NULL when there are none)
+ * The plan equals the one the reference computes with MPI: neighbours,
+ * send/recv counts and offsets (send offsets shifted by local_size,
+ * L2GMap.cpp:119-121) and indexbuf (owner-local indices to pack).
+ * Errors: "Ghosts must be sorted", "Ghost index in local range",
+ * "Ghost index outside the global range".
+ */
+int spmvx_l2gmap_create(int32_t nranks, int32_t rank, const int64_t* local_sizes,
+                        const int64_t* ghosts, int64_t nghosts,
+                        const int64_t* remote_ptr, const int64_t* remote_ghosts,
+                        spmvx_l2gmap* out);
+int spmvx_l2gmap_destroy(spmvx_l2gmap map);
+
+/* local_size / num_ghosts / global_size / global_offset (L2GMap.cpp:187-199),
+ * |indexbuf| and the neighbour count.  Any output may be NULL. */
+int spmvx_l2gmap_info(spmvx_l2gmap map, int64_t* local_size, int64_t* num_ghosts,
+                      int64_t* global_size, int64_t* global_offset,
+                      int64_t* num_indices, int32_t* num_neighbours);
+/* Copies the plan arrays out (host); sizes from spmvx_l2gmap_info: the count
+ * arrays have max(num_neighbours,1) entries, offsets one more
+ * (L2GMap.cpp:86-107).  Any output may be NULL. */
+int spmvx_l2gmap_plan(spmvx_l2gmap map, int32_t* neighbours, int32_t* send_count,
+                      int32_t* recv_count, int32_t* send_offset, int32_t* recv_offset,
+                      int32_t* indexbuf);
+/* Ghost global indices (L2GMap.h:75). */
+int spmvx_l2gmap_ghosts(spmvx_l2gmap map, int64_t* ghosts_host);
+/* global_to_local (L2GMap.cpp:172-185); an index neither owned nor ghosted
+ * is an error (the reference asserts). */
+int spmvx_l2gmap_global_to_local(spmvx_l2gmap map, int64_t global_index, int32_t* local);
+
+/*
+ * Single-process transport.  What the neighbours would send into this rank's
+ * ghost region comes from a host "halo" buffer of num_ghosts values of
+ * `dtype` (F64 or F32), copied into library-owned pinned memory; NULL means
+ * zeros (the default).
+ */
+int spmvx_l2gmap_set_halo(spmvx_l2gmap map, int32_t dtype, const void* halo_host);
+
+/*
+ * L2GMap::update<T> (L2GMap.cpp:126-150) on a device vector of
+ * local_size + num_ghosts values of `dtype`: packs vec[indexbuf[i]] into the
+ * device send buffer (what the neighbours would receive), and unpacks the
+ * halo into vec[local_size ..).  If send_host is not NULL the packed buffer
+ * is also copied there (|indexbuf| values) and the stream is synchronised.
+ */
+int spmvx_l2gmap_update(spmvx_l2gmap map, int32_t dtype, void* vec_dev, void* send_host,
+                        spmvx_stream stream);
+
+/*
+ * L2GMap::reverse_update<T> (L2GMap.cpp:152-170): the ghost region
+ * vec[local_size ..) is what is sent to the owners (copied to ghost_out_host
+ * when not NULL), and incoming_host (|indexbuf| values, what the neighbours
+ * send back; NULL = zeros) is scatter-added: vec[indexbuf[i]] += incoming[i],
+ * repeated indices accumulating in index order.  Synchronises the stream.
+ */
+int spmvx_l2gmap_reverse_update(spmvx_l2gmap map, int32_t dtype, void* vec_dev,
+                                const void* incoming_host, void* ghost_out_host,
+                                spmvx_stream stream);
+
+/* ===================================================================== */
+/* Matrix (spmv/Matrix.h:30-53, spmv/Matrix.cpp:13-211)                   */
+/* ===================================================================== */
+/*
+ * Copies a compressed row-major CSR (rowptr[nrows+1], colidx[nnz] with
+ * 0 <= colidx < ncols, sorted within each row, values[nnz] as double) from
+ * the host to the device, stores the values as `dtype` (F64, F32 or BF16,
+ * round to nearest even) and runs the row-binning analysis (the analogue of
+ * the inspector call mkl_sparse_optimize, Matrix.cpp:43).  Columns
+ * [0, local_size) are owned, [local_size, ncols) ghosts.
+ * Replaces Matrix<T>::Matrix (Matrix.cpp:13-22).
+ */
+int spmvx_matrix_create(int32_t dtype, int64_t nrows, int64_t ncols, int64_t nnz,
+                        const int32_t* rowptr_host, const int32_t* colidx_host,
+                        const double* values_host, spmvx_matrix* out);
+/* A matrix with the same (shared, not copied) pattern as `base` and its
+ * values converted to `dtype` on the device: the low-precision copy used by
+ * mixed-precision refinement. */
+int spmvx_matrix_create_like(spmvx_matrix base, int32_t dtype, spmvx_matrix* out);
+int spmvx_matrix_destroy(spmvx_matrix A);
+int spmvx_matrix_info(spmvx_matrix A, int32_t* dtype, int64_t* nrows, int64_t* ncols,
+                      int64_t* nnz, int64_t* nblocks);
+/* Device pointers of the stored arrays (values in the stored dtype). */
+int spmvx_matrix_arrays(spmvx_matrix A, const int32_t** rowptr_dev,
+                        const int32_t** colidx_dev, const void** values_dev);
+/* Overwrite the values from nnz doubles on the device (pattern unchanged). */
+int spmvx_matrix_set_values(spmvx_matrix A, const double* values_dev, spmvx_stream stream);
+
+/*
+ * y[nrows] = A x[ncols] -- Matrix<T>::operator* (Matrix.h:37-39,
+ * Matrix.cpp:158-166,199-204).  For SPMVX_F64 every row is summed in stored
+ * column order with separately rounded products and sums, starting from 0:
+ * the result is bit-identical to the reference's Eigen product.
+ */
+int spmvx_spmv(spmvx_matrix A, const void* x_dev, void* y_dev, spmvx_stream stream);
+
+/* y[ncols] = A^T x[nrows] -- Matrix<T>::transpmult (Matrix.h:41-43,
+ * Matrix.cpp:206-211).  Uses an analysis-time transposed copy; every output
+ * entry is summed in ascending row order (bit-identical for SPMVX_F64). */
+int spmvx_spmv_t(spmvx_matrix A, const void* x_dev, void* y_dev, spmvx_stream stream);
+
+/* ===================================================================== */
+/* CG (spmv/cg.h:27-29, spmv/cg.cpp:11-83)                                */
+/* ===================================================================== */
+/*
+ * Unpreconditioned CG with x0 = 0, the reference's loop and stopping rule
+ * (||r_k||^2 / ||b||^2 < rtol^2 checked after the direction update, or
+ * k == max_its).  A must be SPMVX_F64 with nrows == col_map local_size.
+ * col_map (may be NULL when A has no ghost columns) is updated on the
+ * direction vector every iteration (cg.cpp:55) through the transport above.
+ *   b_dev[nrows], x_dev[ncols] (ghost entries returned as 0), *its = k.
+ *   hist_host (may be NULL): max_its+1 doubles, hist[j] = ||r_j||^2.
+ * Errors: "spmv::cg - Error: b.rows() != A.rows()" style argument checks.
+ */
+int spmvx_cg(spmvx_matrix A, spmvx_l2gmap col_map, const double* b_dev, double* x_dev,
+             int32_t max_its, double rtol, int32_t* its, double* hist_host,
+             spmvx_stream stream);
+
+/* ===================================================================== */
+/* Extension 1: mixed-precision iterative refinement (no reference        */
+/* counterpart; wraps spmvx_cg's loop as the inner solver)                */
+/* ===================================================================== */
+typedef struct
+{
+  int32_t outer_its;      /* refinement steps taken                          */
+  int32_t inner_its;      /* total inner CG iterations                        */
+  double rel_residual;    /* final ||b - A x|| / ||b|| in FP64                */
+  double outer_hist[64];  /* ||b - A x_j|| / ||b|| before step j (j<=outer)   */
+  int32_t inner_hist[64]; /* inner iterations of step j                       */
+} spmvx_ir_stats;
+
+/*
+ * Solves A64 x = b to ||b - A64 x|| / ||b|| <= rtol.  Each outer step forms
+ * r = b - A64 x (FP64 SpMV) and ||r|| in FP64 on the device, runs CG in FP32
+ * on Alow d = r / ||r|| (Alow: SPMVX_F32 or SPMVX_BF16 copy with the same
+ * pattern; FP32 vectors and accumulation; stop at relative residual
+ * inner_rtol or inner_max_its) and updates x += ||r|| d in FP64.
+ * x_dev[ncols of A64]; ghost columns are treated as zero.
+ */
+int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b_dev, double* x_dev,
+                int32_t max_outer, double rtol, int32_t inner_max_its, double inner_rtol,
+                spmvx_ir_stats* stats, spmvx_stream stream);
+
+/* FP32 CG on an SPMVX_F32 / SPMVX_BF16 matrix (the inner solver alone). */
+int spmvx_cg_f32(spmvx_matrix A, const float* b_dev, float* x_dev, int32_t max_its,
+                 double rtol, int32_t* its, spmvx_stream stream);
+
+/* ===================================================================== */
+/* Extension 3: adjoint helpers (no reference counterpart)                */
+/* ===================================================================== */
+/* SDDMM on the pattern of A: out[k] = alpha * u[row(k)] * v[col(k)], for
+ * F64 matrices (grad_A = -lambda x^T restricted to the pattern). */
+int spmvx_sddmm(spmvx_matrix A, double alpha, const double* u_dev, const double* v_dev,
+                double* out_dev, spmvx_stream stream);
+
+/* ===================================================================== */
+/* Utilities                                                              */
+/* ===================================================================== */
+/* dst[i] = src[i], n doubles: the known-size copy kernel used to calibrate
+ * HBM byte counters. */
+int spmvx_copy_f64(double* dst_dev, const double* src_dev, int64_t n, spmvx_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPMV_AMD_H */

@@ -395,9 +395,10 @@ int64_t or_gen_diffusion(int64_t n, const double* theta, int32_t* rowptr,
       double d = 0.0;
       if (theta)
       {
-        if (e_s >= 0) d += theta[e_s];
+        /* ascending edge index: w < e < s < n < boundary edges */
         if (e_w >= 0) d += theta[e_w];
         if (e_e >= 0) d += theta[e_e];
+        if (e_s >= 0) d += theta[e_s];
         if (e_n >= 0) d += theta[e_n];
         if (j == 0) d += theta[2 * nh + 0 * n + i];
         if (j == n - 1) d += theta[2 * nh + 1 * n + i];

@@ -1,0 +1,298 @@
+// Conjugate gradient on the device (spmv/cg.cpp:11-83) and the
+// mixed-precision iterative refinement built on it.
+//
+// One CG iteration = (ghost update) + 3 kernels, all scalars on the device:
+//   csr_stream<kDot>   y = A p, p.y, alpha = rnorm_old / p.y     cg.cpp:55-62
+//   cg_update_r        r -= alpha y, ||r||^2, beta, k, converged cg.cpp:66-73,78
+//   cg_update_xp       x += alpha p, p = beta p + r, stop flag   cg.cpp:65,76
+// Every kernel returns immediately once the device-side `active` flag is
+// cleared, so the host enqueues iterations in chunks and polls the flag
+// between chunks; the iteration count is exact.
+#include <cmath>
+#include <cstring>
+
+#include "handles.hpp"
+
+namespace spmvx
+{
+namespace
+{
+
+template <typename X, typename Acc>
+void ensure_ws(CgWorkspace& w, const Pattern& P, int64_t ncols)
+{
+  const int64_t n = P.nrows;
+  if (w.mem.ptr && w.n == n && w.ncols == ncols && w.nblocks == P.nblocks && w.elem == sizeof(X))
+    return;
+  const int vgrid = vec_grid(n > ncols ? n : ncols);
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t vec_n = al(sizeof(X) * static_cast<size_t>(n > 0 ? n : 1));
+  const size_t vec_c = al(sizeof(X) * static_cast<size_t>(ncols > 0 ? ncols : 1));
+  const int64_t nb = P.nblocks > 0 ? P.nblocks : 1;
+  const size_t red_s = al(sizeof(Acc) * static_cast<size_t>(nb + red_groups(nb) + 1));
+  const size_t cnt_s = al(sizeof(unsigned) * static_cast<size_t>(1 + red_groups(nb)));
+  const size_t red_v = al(sizeof(Acc) * static_cast<size_t>(vgrid + red_groups(vgrid) + 1));
+  const size_t cnt_v = al(sizeof(unsigned) * static_cast<size_t>(1 + red_groups(vgrid)));
+  const size_t st_s = al(sizeof(CgState<Acc>));
+  const size_t total = 2 * vec_n + 3 * vec_c + red_s + cnt_s + 2 * (red_v + cnt_v) + al(4) + st_s;
+  w.mem.release();
+  char* base = static_cast<char*>(w.mem.ensure(total)); // zeroed: counters start at 0
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    void* p = base + off;
+    off += b;
+    return p;
+  };
+  w.r = take(vec_n);
+  w.y = take(vec_n);
+  w.p = take(vec_c);
+  w.aux = take(vec_c);
+  w.aux2 = take(vec_c);
+  w.red_spmv = take(red_s);
+  w.cnt_spmv = static_cast<unsigned*>(take(cnt_s));
+  w.red_vec0 = take(red_v);
+  w.cnt_vec0 = static_cast<unsigned*>(take(cnt_v));
+  w.red_vec1 = take(red_v);
+  w.cnt_vec1 = static_cast<unsigned*>(take(cnt_v));
+  w.cnt_xp = static_cast<unsigned*>(take(al(4)));
+  w.state = take(st_s);
+  w.n = n;
+  w.ncols = ncols;
+  w.nblocks = P.nblocks;
+  w.elem = sizeof(X);
+  w.vgrid = vgrid;
+}
+
+// Host driver of one CG solve.  x has ncols entries; b has nrows.
+template <typename V, typename X, typename Acc>
+int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t vec_dtype,
+           const X* b, X* x, int32_t kmax, Acc rtol, Acc* hist_dev, hipStream_t s)
+{
+  const Pattern& P = *A->pat;
+  const int64_t n = P.nrows, ncols = P.ncols;
+  auto* st = static_cast<CgState<Acc>*>(w.state);
+  CgState<Acc> init{};
+  init.rtol2 = rtol * rtol;
+  init.kmax = kmax;
+  SPMVX_HIP(hipMemcpyAsync(st, &init, sizeof(init), hipMemcpyHostToDevice, s));
+  X* r = static_cast<X*>(w.r);
+  X* y = static_cast<X*>(w.y);
+  X* p = static_cast<X*>(w.p);
+  RedWs<Acc> rs{static_cast<Acc*>(w.red_spmv), w.cnt_spmv};
+  RedWs<Acc> r0{static_cast<Acc*>(w.red_vec0), w.cnt_vec0};
+  RedWs<Acc> r1{static_cast<Acc*>(w.red_vec1), w.cnt_vec1};
+  const int vg = vec_grid(n);
+  cg_init<X, Acc><<<vec_grid(ncols > n ? ncols : n), kVecBS, 0, s>>>(n, ncols, b, x, r, p, r0, st);
+  check_launch("cg_init");
+  if (hist_dev)
+  {
+    // hist[0] = rnorm0 (read back with the state at the end)
+  }
+  auto* flag = static_cast<CgState<Acc>*>(A->host_flag.ensure(sizeof(CgState<Acc>)));
+  int32_t launched = 0;
+  int32_t chunk = 4;
+  while (launched < kmax)
+  {
+    const int32_t nb = (kmax - launched) < chunk ? (kmax - launched) : chunk;
+    for (int32_t j = 0; j < nb; ++j)
+    {
+      if (col_map)
+        l2gmap_update_async(col_map, vec_dtype, p, s); // cg.cpp:55
+      if (P.nblocks > 0)
+      {
+        csr_stream<V, X, Acc, kDot><<<dim3(static_cast<unsigned>(P.nblocks)), dim3(kBS), 0, s>>>(
+            P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(), static_cast<const V*>(A->values.ptr),
+            p, y, P.blocks.as<int32_t>(), P.nblocks, nullptr, rs, st);
+        check_launch("csr_stream<kDot>");
+      }
+      cg_update_r<X, Acc><<<vg, kVecBS, 0, s>>>(n, r, y, r1, st, hist_dev);
+      cg_update_xp<X, Acc><<<vg, kVecBS, 0, s>>>(n, x, p, r, w.cnt_xp, st);
+    }
+    check_launch("cg iteration");
+    launched += nb;
+    SPMVX_HIP(hipMemcpyAsync(flag, st, sizeof(CgState<Acc>), hipMemcpyDeviceToHost, s));
+    SPMVX_HIP(hipStreamSynchronize(s));
+    if (!flag->active)
+      break;
+    if (chunk < 32)
+      chunk *= 2;
+  }
+  SPMVX_HIP(hipMemcpyAsync(flag, st, sizeof(CgState<Acc>), hipMemcpyDeviceToHost, s));
+  SPMVX_HIP(hipStreamSynchronize(s));
+  if (hist_dev)
+    SPMVX_HIP(hipMemcpy(hist_dev, &flag->rnorm0, sizeof(Acc), hipMemcpyHostToDevice));
+  return flag->k;
+}
+
+// ---------------------------------------------------------------- refinement
+// x = 0 (FP64)
+__global__ void zero_f64(double* x, int64_t n)
+{
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+    x[i] = 0.0;
+}
+
+// r32 = (float)(r64 * scale)
+__global__ void scale_to_f32(const double* __restrict__ r, float* __restrict__ r32, int64_t n,
+                             const CgState<double>* st)
+{
+  const double scale = 1.0 / sqrt(st->last);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+    r32[i] = static_cast<float>(r[i] * scale);
+}
+
+// x64 += norm * (double)d32
+__global__ void add_correction(double* __restrict__ x, const float* __restrict__ d, int64_t n,
+                               const CgState<double>* st)
+{
+  const double norm = sqrt(st->last);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+    x[i] += norm * static_cast<double>(d[i]);
+}
+
+} // namespace
+} // namespace spmvx
+
+using namespace spmvx;
+
+extern "C" {
+
+int spmvx_cg(spmvx_matrix A, spmvx_l2gmap col_map, const double* b, double* x, int32_t max_its,
+             double rtol, int32_t* its, double* hist_host, spmvx_stream stream)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    SPMVX_REQUIRE(A->dtype == SPMVX_F64, "spmv::cg needs an FP64 matrix");
+    const Pattern& P = *A->pat;
+    if (col_map)
+    {
+      SPMVX_REQUIRE(col_map->local_size == P.nrows,
+                    "spmv::cg - Error: b.rows() != A.rows()");
+      SPMVX_REQUIRE(col_map->local_size + static_cast<int64_t>(col_map->ghosts.size()) == P.ncols,
+                    "column map size differs from the matrix column count");
+    }
+    else
+      SPMVX_REQUIRE(P.ncols == P.nrows, "a matrix with ghost columns needs its column map");
+    SPMVX_REQUIRE(max_its >= 0, "max_its must be >= 0");
+    SPMVX_REQUIRE((b && x) || P.nrows == 0, "NULL vector");
+    hipStream_t s = as_stream(stream);
+    ensure_ws<double, double>(A->ws64, P, P.ncols);
+    DevBuf hist;
+    double* hd = nullptr;
+    if (hist_host)
+    {
+      hd = static_cast<double*>(hist.ensure(sizeof(double) * (static_cast<size_t>(max_its) + 1)));
+    }
+    const int k = run_cg<double, double, double>(A, A->ws64, col_map, SPMVX_F64, b, x, max_its,
+                                                 rtol, hd, s);
+    if (its)
+      *its = k;
+    if (hist_host)
+      SPMVX_HIP(hipMemcpy(hist_host, hd, sizeof(double) * (static_cast<size_t>(k) + 1),
+                          hipMemcpyDeviceToHost));
+  });
+}
+
+int spmvx_cg_f32(spmvx_matrix A, const float* b, float* x, int32_t max_its, double rtol,
+                 int32_t* its, spmvx_stream stream)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    SPMVX_REQUIRE(A->dtype == SPMVX_F32 || A->dtype == SPMVX_BF16,
+                  "cg_f32 needs an F32 or BF16 matrix");
+    const Pattern& P = *A->pat;
+    SPMVX_REQUIRE(P.ncols == P.nrows, "cg_f32 needs a square matrix without ghost columns");
+    SPMVX_REQUIRE(max_its >= 0, "max_its must be >= 0");
+    hipStream_t s = as_stream(stream);
+    ensure_ws<float, float>(A->ws32, P, P.ncols);
+    int k;
+    if (A->dtype == SPMVX_F32)
+      k = run_cg<float, float, float>(A, A->ws32, nullptr, SPMVX_F32, b, x, max_its,
+                                      static_cast<float>(rtol), nullptr, s);
+    else
+      k = run_cg<bf16_t, float, float>(A, A->ws32, nullptr, SPMVX_F32, b, x, max_its,
+                                       static_cast<float>(rtol), nullptr, s);
+    if (its)
+      *its = k;
+  });
+}
+
+int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
+                int32_t max_outer, double rtol, int32_t inner_max_its, double inner_rtol,
+                spmvx_ir_stats* stats, spmvx_stream stream)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A64 && Alow, "NULL matrix");
+    SPMVX_REQUIRE(A64->dtype == SPMVX_F64, "outer matrix must be FP64");
+    SPMVX_REQUIRE(Alow->dtype == SPMVX_F32 || Alow->dtype == SPMVX_BF16,
+                  "inner matrix must be F32 or BF16");
+    SPMVX_REQUIRE(Alow->pat == A64->pat, "inner matrix must share the pattern (create_like)");
+    const Pattern& P = *A64->pat;
+    SPMVX_REQUIRE(P.nrows == P.ncols, "refinement needs a square matrix");
+    SPMVX_REQUIRE(max_outer >= 0 && max_outer <= 64, "max_outer must be in [0, 64]");
+    hipStream_t s = as_stream(stream);
+    const int64_t n = P.nrows;
+    ensure_ws<double, double>(A64->ws64, P, P.ncols);
+    ensure_ws<float, float>(Alow->ws32, P, P.ncols);
+    CgWorkspace& w = A64->ws64;
+    auto* st = static_cast<CgState<double>*>(w.state);
+    CgState<double> init{};
+    init.active = 1;
+    SPMVX_HIP(hipMemcpyAsync(st, &init, sizeof(init), hipMemcpyHostToDevice, s));
+    double* r = static_cast<double*>(w.r);
+    float* r32 = static_cast<float*>(Alow->ws32.aux);
+    float* d32 = static_cast<float*>(Alow->ws32.aux2);
+    RedWs<double> rs{static_cast<double*>(w.red_spmv), w.cnt_spmv};
+    const int vg = vec_grid(n);
+    zero_f64<<<vg, kVecBS, 0, s>>>(x, n);
+    spmvx_ir_stats local{};
+    auto* flag = static_cast<CgState<double>*>(A64->host_flag.ensure(sizeof(CgState<double>)));
+    double bnorm = 0.0;
+    int32_t outer = 0;
+    for (;;)
+    {
+      // r = b - A x and ||r||^2 in FP64
+      if (P.nblocks > 0)
+        csr_stream<double, double, double, kResid>
+            <<<dim3(static_cast<unsigned>(P.nblocks)), dim3(kBS), 0, s>>>(
+                P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
+                static_cast<const double*>(A64->values.ptr), x, r, P.blocks.as<int32_t>(),
+                P.nblocks, b, rs, st);
+      check_launch("csr_stream<kResid>");
+      SPMVX_HIP(hipMemcpyAsync(flag, st, sizeof(CgState<double>), hipMemcpyDeviceToHost, s));
+      SPMVX_HIP(hipStreamSynchronize(s));
+      const double rn = std::sqrt(flag->last);
+      if (outer == 0)
+        bnorm = rn;
+      const double rel = bnorm > 0 ? rn / bnorm : 0.0;
+      if (outer < 64)
+        local.outer_hist[outer] = rel;
+      local.rel_residual = rel;
+      if (rel <= rtol || outer >= max_outer || rn == 0.0)
+        break;
+      // inner FP32 solve on the normalised residual
+      scale_to_f32<<<vg, kVecBS, 0, s>>>(r, r32, n, st);
+      check_launch("scale_to_f32");
+      int k;
+      if (Alow->dtype == SPMVX_F32)
+        k = run_cg<float, float, float>(Alow, Alow->ws32, nullptr, SPMVX_F32, r32, d32,
+                                        inner_max_its, static_cast<float>(inner_rtol), nullptr, s);
+      else
+        k = run_cg<bf16_t, float, float>(Alow, Alow->ws32, nullptr, SPMVX_F32, r32, d32,
+                                         inner_max_its, static_cast<float>(inner_rtol), nullptr, s);
+      local.inner_hist[outer] = k;
+      local.inner_its += k;
+      add_correction<<<vg, kVecBS, 0, s>>>(x, d32, n, st);
+      check_launch("add_correction");
+      ++outer;
+    }
+    local.outer_its = outer;
+    if (stats)
+      *stats = local;
+  });
+}
+
+} // extern "C"

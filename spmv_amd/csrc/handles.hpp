@@ -1,0 +1,165 @@
+// Internal definitions of the opaque handles of the C ABI.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace spmvx
+{
+
+// Device allocation owned by a handle.
+struct DevBuf
+{
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release()
+  {
+    if (ptr)
+      (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  // Grows (never shrinks) to n bytes; new memory is zeroed.
+  void* ensure(size_t n)
+  {
+    if (n > bytes)
+    {
+      release();
+      SPMVX_HIP(hipMalloc(&ptr, n));
+      SPMVX_HIP(hipMemset(ptr, 0, n));
+      bytes = n;
+    }
+    return ptr;
+  }
+  template <typename T>
+  T* as() const
+  {
+    return static_cast<T*>(ptr);
+  }
+};
+
+// Pinned host allocation (hipHostMalloc).
+struct PinnedBuf
+{
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf()
+  {
+    if (ptr)
+      (void)hipHostFree(ptr);
+  }
+  void* ensure(size_t n)
+  {
+    if (n > bytes)
+    {
+      if (ptr)
+        (void)hipHostFree(ptr);
+      ptr = nullptr;
+      SPMVX_HIP(hipHostMalloc(&ptr, n, hipHostMallocDefault));
+      bytes = n;
+    }
+    return ptr;
+  }
+};
+
+// CSR pattern + row-binning analysis on the device; shared between a matrix
+// and its reduced-precision copies.
+struct Pattern
+{
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  DevBuf rowptr, colidx, blocks;
+  int64_t nblocks = 0;
+  int device = 0;
+};
+
+// Row blocks for the ordered stream kernel: consecutive rows, at most
+// max_rows rows and max_nnz entries per block; a row longer than max_nnz
+// is a block of its own.
+std::vector<int32_t> row_blocks(const int32_t* rowptr, int64_t nrows, int64_t max_nnz,
+                                int64_t max_rows);
+
+// Number of groups of 256 used by grid_reduce for nb blocks.
+inline int64_t red_groups(int64_t nb) { return (nb + 255) / 256; }
+
+// Grid of the vector kernels for n elements.
+inline int vec_grid(int64_t n)
+{
+  int64_t g = (n + kVecBS - 1) / kVecBS;
+  if (g > 4096)
+    g = 4096;
+  if (g < 1)
+    g = 1;
+  return static_cast<int>(g);
+}
+
+// Per-precision scratch of the CG solvers, owned by a matrix handle.
+struct CgWorkspace
+{
+  DevBuf mem;
+  int64_t n = 0, ncols = 0, nblocks = 0;
+  size_t elem = 0;
+  // carved pointers
+  void* r = nullptr;
+  void* y = nullptr;
+  void* p = nullptr;
+  void* aux = nullptr; // extra vector (refinement: residual / correction)
+  void* aux2 = nullptr;
+  void* state = nullptr;
+  void* red_spmv = nullptr;
+  unsigned* cnt_spmv = nullptr;
+  void* red_vec0 = nullptr;
+  unsigned* cnt_vec0 = nullptr;
+  void* red_vec1 = nullptr;
+  unsigned* cnt_vec1 = nullptr;
+  unsigned* cnt_xp = nullptr;
+  int vgrid = 0;
+};
+
+} // namespace spmvx
+
+struct spmvx_matrix_s
+{
+  int32_t dtype = SPMVX_F64;
+  std::shared_ptr<spmvx::Pattern> pat;
+  spmvx::DevBuf values;
+  // transposed copy for spmv_t (built on first use)
+  std::shared_ptr<spmvx::Pattern> tpat;
+  spmvx::DevBuf tvalues;
+  // solver scratch (FP64 / FP32)
+  spmvx::CgWorkspace ws64, ws32;
+  spmvx::PinnedBuf host_flag;
+};
+
+struct spmvx_l2gmap_s
+{
+  int32_t nranks = 1, rank = 0;
+  std::vector<int64_t> ranges;
+  std::vector<int64_t> ghosts;
+  std::vector<int32_t> neighbours, send_count, recv_count, send_offset, recv_offset;
+  std::vector<int32_t> indexbuf;
+  // deterministic scatter-add plan of reverse_update: unique targets and the
+  // positions in indexbuf (ascending) that add into each
+  std::vector<int32_t> rev_target, rev_ptr, rev_pos;
+  int64_t local_size = 0;
+  // device side (allocated on first use)
+  spmvx::DevBuf d_indexbuf, d_send, d_inbox, d_rev_target, d_rev_ptr, d_rev_pos;
+  spmvx::PinnedBuf h_halo, h_send, h_in;
+  int32_t halo_dtype = -1; // dtype of the halo in h_halo, -1 = zeros
+  bool device_ready = false;
+};
+
+namespace spmvx
+{
+// Device plan upload (l2gmap.hip) and the update used by the solvers.
+void l2gmap_prepare_device(spmvx_l2gmap_s* m);
+void l2gmap_update_async(spmvx_l2gmap_s* m, int32_t dtype, void* vec_dev, hipStream_t s);
+} // namespace spmvx

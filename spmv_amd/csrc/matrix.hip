@@ -1,0 +1,404 @@
+// Matrix handle: CSR upload + row-binning analysis, SpMV, transpose SpMV,
+// SDDMM, value updates (spmv/Matrix.h:30-53, spmv/Matrix.cpp:13-211).
+#include <cmath>
+#include <cstring>
+
+#include "handles.hpp"
+
+namespace spmvx
+{
+
+std::vector<int32_t> row_blocks(const int32_t* rowptr, int64_t nrows, int64_t max_nnz,
+                                int64_t max_rows)
+{
+  std::vector<int32_t> b;
+  b.reserve(static_cast<size_t>(nrows / 16 + 2));
+  b.push_back(0);
+  int64_t r = 0;
+  while (r < nrows)
+  {
+    const int64_t start = r;
+    int64_t acc = 0;
+    while (r < nrows && r - start < max_rows)
+    {
+      const int64_t len = rowptr[r + 1] - rowptr[r];
+      if (acc + len > max_nnz)
+        break;
+      acc += len;
+      ++r;
+    }
+    if (r == start)
+      ++r; // a single row longer than max_nnz
+    b.push_back(static_cast<int32_t>(r));
+  }
+  return b;
+}
+
+namespace
+{
+
+// double -> bf16, round to nearest even, via round-to-odd to float (exact
+// double rounding: float keeps >= 2 more bits than bf16).
+uint16_t host_f64_to_bf16(double d)
+{
+  if (std::isnan(d))
+    return 0x7FC0;
+  float f = static_cast<float>(d);
+  if (std::isfinite(f) && static_cast<double>(f) != d)
+  {
+    if (std::fabs(static_cast<double>(f)) > std::fabs(d))
+      f = std::nextafter(f, 0.0f);
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u |= 1u;
+    std::memcpy(&f, &u, 4);
+  }
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+
+__device__ __forceinline__ uint16_t dev_f64_to_bf16(double d)
+{
+  if (d != d)
+    return 0x7FC0;
+  float f = __double2float_rz(d);
+  uint32_t u = __float_as_uint(f);
+  if (static_cast<double>(f) != d && (u & 0x7F800000u) != 0x7F800000u)
+    u |= 1u;
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+
+size_t dtype_size(int32_t dtype)
+{
+  switch (dtype)
+  {
+  case SPMVX_F64: return 8;
+  case SPMVX_F32: return 4;
+  case SPMVX_BF16: return 2;
+  default: throw Error(SPMVX_ERR_ARG, "unknown dtype");
+  }
+}
+
+__global__ void convert_values(const double* __restrict__ src, void* __restrict__ dst,
+                               int64_t n, int32_t dtype)
+{
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+  {
+    const double d = src[i];
+    if (dtype == SPMVX_F64)
+      static_cast<double*>(dst)[i] = d;
+    else if (dtype == SPMVX_F32)
+      static_cast<float*>(dst)[i] = static_cast<float>(d);
+    else
+      static_cast<bf16_t*>(dst)[i].bits = dev_f64_to_bf16(d);
+  }
+}
+
+__global__ void copy_f64_kernel(double* __restrict__ dst, const double* __restrict__ src, int64_t n)
+{
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const int64_t n2 = n / 2;
+  const double2* s2 = reinterpret_cast<const double2*>(src);
+  double2* d2 = reinterpret_cast<double2*>(dst);
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n2; i += stride)
+    d2[i] = s2[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1))
+    dst[n - 1] = src[n - 1];
+}
+
+// out[k] = alpha * u[row(k)] * v[col(k)]; one workgroup per row block, rows of
+// an entry found by binary search in the block's row pointers (LDS).
+__global__ __launch_bounds__(kBS) void sddmm_kernel(const int32_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ colidx,
+                                                    const int32_t* __restrict__ blocks,
+                                                    int64_t nblocks, double alpha,
+                                                    const double* __restrict__ u,
+                                                    const double* __restrict__ v,
+                                                    double* __restrict__ out)
+{
+  __shared__ int32_t rp[kBS + 1];
+  const int64_t bi = xcd_block(blockIdx.x, nblocks);
+  const int32_t r0 = blocks[bi], r1 = blocks[bi + 1];
+  const int nr = r1 - r0;
+  for (int t = threadIdx.x; t <= nr; t += kBS)
+    rp[t] = rowptr[r0 + t];
+  __syncthreads();
+  const int32_t k0 = rp[0], k1 = rp[nr];
+  for (int32_t k = k0 + threadIdx.x; k < k1; k += kBS)
+  {
+    int lo = 0, hi = nr - 1; // last row with rp[row] <= k
+    while (lo < hi)
+    {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rp[mid] <= k)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    out[k] = (alpha * u[r0 + lo]) * v[colidx[k]];
+  }
+}
+
+template <typename V, typename X, typename Acc>
+void launch_plain(const Pattern& P, const void* vals, const void* x, void* y, hipStream_t s)
+{
+  if (P.nblocks == 0)
+    return;
+  csr_stream<V, X, Acc, kPlain><<<dim3(static_cast<unsigned>(P.nblocks)), dim3(kBS), 0, s>>>(
+      P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(), static_cast<const V*>(vals),
+      static_cast<const X*>(x), static_cast<X*>(y), P.blocks.as<int32_t>(), P.nblocks,
+      nullptr, RedWs<Acc>{nullptr, nullptr}, nullptr);
+  check_launch("csr_stream");
+}
+
+void launch_spmv(int32_t dtype, const Pattern& P, const void* vals, const void* x, void* y,
+                 hipStream_t s)
+{
+  if (dtype == SPMVX_F64)
+    launch_plain<double, double, double>(P, vals, x, y, s);
+  else if (dtype == SPMVX_F32)
+    launch_plain<float, float, float>(P, vals, x, y, s);
+  else
+    launch_plain<bf16_t, float, float>(P, vals, x, y, s);
+}
+
+std::shared_ptr<Pattern> upload_pattern(int64_t nrows, int64_t ncols, int64_t nnz,
+                                        const int32_t* rowptr, const int32_t* colidx)
+{
+  auto P = std::make_shared<Pattern>();
+  P->nrows = nrows;
+  P->ncols = ncols;
+  P->nnz = nnz;
+  SPMVX_HIP(hipGetDevice(&P->device));
+  P->rowptr.ensure(sizeof(int32_t) * static_cast<size_t>(nrows + 1));
+  SPMVX_HIP(hipMemcpy(P->rowptr.ptr, rowptr, sizeof(int32_t) * (nrows + 1), hipMemcpyHostToDevice));
+  P->colidx.ensure(sizeof(int32_t) * static_cast<size_t>(nnz > 0 ? nnz : 1));
+  if (nnz > 0)
+    SPMVX_HIP(hipMemcpy(P->colidx.ptr, colidx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+  std::vector<int32_t> b = row_blocks(rowptr, nrows, kChunk, kBS);
+  P->nblocks = static_cast<int64_t>(b.size()) - 1;
+  P->blocks.ensure(sizeof(int32_t) * b.size());
+  SPMVX_HIP(hipMemcpy(P->blocks.ptr, b.data(), sizeof(int32_t) * b.size(), hipMemcpyHostToDevice));
+  return P;
+}
+
+void validate_csr(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* rowptr,
+                  const int32_t* colidx)
+{
+  SPMVX_REQUIRE(nrows >= 0 && ncols >= 0 && nnz >= 0, "negative matrix size");
+  SPMVX_REQUIRE(nrows < (int64_t(1) << 31) && ncols < (int64_t(1) << 31)
+                    && nnz < (int64_t(1) << 31),
+                "matrix too large for 32-bit indices");
+  SPMVX_REQUIRE(rowptr != nullptr, "rowptr is NULL");
+  SPMVX_REQUIRE(nnz == 0 || colidx != nullptr, "colidx is NULL");
+  SPMVX_REQUIRE(rowptr[0] == 0, "rowptr[0] must be 0");
+  SPMVX_REQUIRE(rowptr[nrows] == nnz, "rowptr[nrows] must equal nnz");
+  for (int64_t i = 0; i < nrows; ++i)
+    SPMVX_REQUIRE(rowptr[i + 1] >= rowptr[i], "rowptr must be non-decreasing");
+  for (int64_t k = 0; k < nnz; ++k)
+    SPMVX_REQUIRE(colidx[k] >= 0 && colidx[k] < ncols, "column index out of range");
+}
+
+void build_transpose(spmvx_matrix_s* A)
+{
+  const Pattern& P = *A->pat;
+  const size_t es = dtype_size(A->dtype);
+  std::vector<int32_t> rp(P.nrows + 1), ci(P.nnz);
+  std::vector<uint8_t> v(es * P.nnz);
+  SPMVX_HIP(hipMemcpy(rp.data(), P.rowptr.ptr, sizeof(int32_t) * (P.nrows + 1), hipMemcpyDeviceToHost));
+  if (P.nnz)
+  {
+    SPMVX_HIP(hipMemcpy(ci.data(), P.colidx.ptr, sizeof(int32_t) * P.nnz, hipMemcpyDeviceToHost));
+    SPMVX_HIP(hipMemcpy(v.data(), A->values.ptr, es * P.nnz, hipMemcpyDeviceToHost));
+  }
+  // counting sort by column; rows ascending within each column
+  std::vector<int32_t> trp(P.ncols + 1, 0), tci(P.nnz);
+  std::vector<uint8_t> tv(es * P.nnz);
+  for (int64_t k = 0; k < P.nnz; ++k)
+    ++trp[ci[k] + 1];
+  for (int64_t c = 0; c < P.ncols; ++c)
+    trp[c + 1] += trp[c];
+  std::vector<int32_t> next(trp.begin(), trp.end() - 1);
+  for (int64_t r = 0; r < P.nrows; ++r)
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k)
+    {
+      const int32_t dst = next[ci[k]]++;
+      tci[dst] = static_cast<int32_t>(r);
+      std::memcpy(&tv[es * dst], &v[es * k], es);
+    }
+  A->tpat = upload_pattern(P.ncols, P.nrows, P.nnz, trp.data(), tci.data());
+  A->tvalues.ensure(es * static_cast<size_t>(P.nnz > 0 ? P.nnz : 1));
+  if (P.nnz)
+    SPMVX_HIP(hipMemcpy(A->tvalues.ptr, tv.data(), es * P.nnz, hipMemcpyHostToDevice));
+}
+
+} // namespace
+} // namespace spmvx
+
+using namespace spmvx;
+
+extern "C" {
+
+int spmvx_matrix_create(int32_t dtype, int64_t nrows, int64_t ncols, int64_t nnz,
+                        const int32_t* rowptr, const int32_t* colidx, const double* values,
+                        spmvx_matrix* out)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(out != nullptr, "out is NULL");
+    *out = nullptr;
+    const size_t es = dtype_size(dtype);
+    validate_csr(nrows, ncols, nnz, rowptr, colidx);
+    SPMVX_REQUIRE(nnz == 0 || values != nullptr, "values is NULL");
+    auto A = std::make_unique<spmvx_matrix_s>();
+    A->dtype = dtype;
+    A->pat = upload_pattern(nrows, ncols, nnz, rowptr, colidx);
+    A->values.ensure(es * static_cast<size_t>(nnz > 0 ? nnz : 1));
+    if (nnz > 0)
+    {
+      if (dtype == SPMVX_F64)
+        SPMVX_HIP(hipMemcpy(A->values.ptr, values, 8 * nnz, hipMemcpyHostToDevice));
+      else if (dtype == SPMVX_F32)
+      {
+        std::vector<float> f(values, values + nnz);
+        SPMVX_HIP(hipMemcpy(A->values.ptr, f.data(), 4 * nnz, hipMemcpyHostToDevice));
+      }
+      else
+      {
+        std::vector<uint16_t> h(static_cast<size_t>(nnz));
+        for (int64_t k = 0; k < nnz; ++k)
+          h[k] = host_f64_to_bf16(values[k]);
+        SPMVX_HIP(hipMemcpy(A->values.ptr, h.data(), 2 * nnz, hipMemcpyHostToDevice));
+      }
+    }
+    *out = A.release();
+  });
+}
+
+int spmvx_matrix_create_like(spmvx_matrix base, int32_t dtype, spmvx_matrix* out)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(base != nullptr && out != nullptr, "NULL handle");
+    SPMVX_REQUIRE(base->dtype == SPMVX_F64, "create_like needs an FP64 base matrix");
+    *out = nullptr;
+    const size_t es = dtype_size(dtype);
+    auto A = std::make_unique<spmvx_matrix_s>();
+    A->dtype = dtype;
+    A->pat = base->pat;
+    const int64_t nnz = base->pat->nnz;
+    A->values.ensure(es * static_cast<size_t>(nnz > 0 ? nnz : 1));
+    if (nnz > 0)
+    {
+      convert_values<<<1024, 256>>>(base->values.as<double>(), A->values.ptr, nnz, dtype);
+      check_launch("convert_values");
+      SPMVX_HIP(hipDeviceSynchronize());
+    }
+    *out = A.release();
+  });
+}
+
+int spmvx_matrix_destroy(spmvx_matrix A)
+{
+  return guarded([&] { delete A; });
+}
+
+int spmvx_matrix_info(spmvx_matrix A, int32_t* dtype, int64_t* nrows, int64_t* ncols,
+                      int64_t* nnz, int64_t* nblocks)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    if (dtype) *dtype = A->dtype;
+    if (nrows) *nrows = A->pat->nrows;
+    if (ncols) *ncols = A->pat->ncols;
+    if (nnz) *nnz = A->pat->nnz;
+    if (nblocks) *nblocks = A->pat->nblocks;
+  });
+}
+
+int spmvx_matrix_arrays(spmvx_matrix A, const int32_t** rowptr, const int32_t** colidx,
+                        const void** values)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    if (rowptr) *rowptr = A->pat->rowptr.as<int32_t>();
+    if (colidx) *colidx = A->pat->colidx.as<int32_t>();
+    if (values) *values = A->values.ptr;
+  });
+}
+
+int spmvx_matrix_set_values(spmvx_matrix A, const double* values_dev, spmvx_stream stream)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    const int64_t nnz = A->pat->nnz;
+    SPMVX_REQUIRE(nnz == 0 || values_dev != nullptr, "values is NULL");
+    if (nnz > 0)
+    {
+      convert_values<<<1024, 256, 0, as_stream(stream)>>>(values_dev, A->values.ptr, nnz, A->dtype);
+      check_launch("convert_values");
+    }
+    A->tpat.reset(); // transposed copy is stale
+    A->tvalues.release();
+  });
+}
+
+int spmvx_spmv(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    SPMVX_REQUIRE(A->pat->nrows == 0 || y != nullptr, "y is NULL");
+    SPMVX_REQUIRE(A->pat->nnz == 0 || x != nullptr, "x is NULL");
+    launch_spmv(A->dtype, *A->pat, A->values.ptr, x, y, as_stream(stream));
+  });
+}
+
+int spmvx_spmv_t(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    if (!A->tpat)
+      build_transpose(A);
+    SPMVX_REQUIRE(A->pat->ncols == 0 || y != nullptr, "y is NULL");
+    launch_spmv(A->dtype, *A->tpat, A->tvalues.ptr, x, y, as_stream(stream));
+  });
+}
+
+int spmvx_sddmm(spmvx_matrix A, double alpha, const double* u, const double* v, double* out,
+                spmvx_stream stream)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    SPMVX_REQUIRE(A->dtype == SPMVX_F64, "sddmm needs an FP64 matrix");
+    const Pattern& P = *A->pat;
+    if (P.nblocks == 0 || P.nnz == 0)
+      return;
+    SPMVX_REQUIRE(u && v && out, "NULL vector");
+    sddmm_kernel<<<dim3(static_cast<unsigned>(P.nblocks)), dim3(kBS), 0, as_stream(stream)>>>(
+        P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(), P.blocks.as<int32_t>(), P.nblocks, alpha,
+        u, v, out);
+    check_launch("sddmm_kernel");
+  });
+}
+
+int spmvx_copy_f64(double* dst, const double* src, int64_t n, spmvx_stream stream)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(n >= 0, "negative size");
+    if (n == 0)
+      return;
+    SPMVX_REQUIRE(dst && src, "NULL pointer");
+    SPMVX_REQUIRE((reinterpret_cast<uintptr_t>(dst) & 15) == 0
+                      && (reinterpret_cast<uintptr_t>(src) & 15) == 0,
+                  "copy needs 16-byte aligned pointers");
+    int64_t grid = (n / 2 + 255) / 256;
+    if (grid > 8192) grid = 8192;
+    if (grid < 1) grid = 1;
+    copy_f64_kernel<<<dim3(static_cast<unsigned>(grid)), dim3(256), 0, as_stream(stream)>>>(dst, src, n);
+    check_launch("copy_f64_kernel");
+  });
+}
+
+} // extern "C"

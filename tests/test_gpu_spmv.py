@@ -1,0 +1,183 @@
+"""GPU parity: SpMV, transpose SpMV, L2GMap update / reverse_update and SDDMM
+through the C ABI against the CPU oracle.  FP64 results must be bit-identical
+(the reference's Eigen row-major product sums every row sequentially with
+separately rounded products and sums; so does the oracle)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import spmv_amd as S
+from spmv_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=DEV, dtype=dtype)
+
+
+def _ocsr(A):
+    return O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
+
+
+def long_row_matrix(n=6000, seed=4):
+    """Rows of very different lengths: a dense first row (> the 2048-entry
+    LDS chunk), empty rows, and short random rows."""
+    rng = np.random.default_rng(seed)
+    rows = [np.arange(n, dtype=np.int32)]
+    for i in range(1, n):
+        if i % 7 == 0:
+            rows.append(np.zeros(0, np.int32))
+        elif i % 1001 == 0:
+            rows.append(np.sort(rng.choice(n, 3000, replace=False)).astype(np.int32))
+        else:
+            rows.append(np.sort(rng.choice(n, rng.integers(1, 40), replace=False)).astype(np.int32))
+    rowptr = np.zeros(n + 1, np.int64)
+    rowptr[1:] = np.cumsum([len(r) for r in rows])
+    colidx = np.concatenate(rows)
+    val = P.uniform(seed, len(colidx), -1, 1)
+    return P.CSR(n, n, rowptr.astype(np.int32), colidx, val)
+
+
+def matrices():
+    yield "demo1d", P.demo1d(64)
+    yield "poisson5", P.stencil(5, 33, 31)
+    yield "poisson7", P.stencil(7, 13, 12, 11)
+    yield "stencil27_ghost", P.stencil(27, 11, 12, 13, ghost_planes=1)
+    yield "diffusion", P.diffusion(40, np.exp(0.5 * P.normal(3, P.diffusion_num_edges(40))))
+    yield "random_spd", P.random_spd(5000, 11)
+    yield "long_rows", long_row_matrix()
+    yield "one_row", P.CSR(1, 3, np.array([0, 3], np.int32), np.array([0, 1, 2], np.int32),
+                           np.array([1.0, -2.0, 3.0]))
+
+
+MATS = list(matrices())
+
+
+@pytest.mark.parametrize("name,A", MATS, ids=[m[0] for m in MATS])
+def test_spmv_f64_bitexact(name, A):
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    x = P.uniform(7, A.ncols, -1, 1)
+    y = M.mult(_dev(x)).cpu().numpy()
+    assert np.array_equal(y, O.spmv(_ocsr(A), x))
+
+
+@pytest.mark.parametrize("name,A", MATS, ids=[m[0] for m in MATS])
+def test_spmv_transpose_bitexact(name, A):
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    x = P.uniform(8, A.nrows, -1, 1)
+    yt = M.transpmult(_dev(x)).cpu().numpy()
+    assert np.array_equal(yt, O.spmv_t(_ocsr(A), x))
+
+
+def _bf16_round(v):
+    """double -> bf16 (round to nearest even) -> float32, on the host."""
+    f = v.astype(np.float32)
+    # exact double rounding via round-to-odd in float
+    back = f.astype(np.float64)
+    bits = f.view(np.uint32).copy()
+    inexact = back != v
+    toward = np.abs(back) > np.abs(v)
+    bits = np.where(inexact & toward, bits - 1, bits)
+    bits = np.where(inexact, bits | 1, bits).astype(np.uint32)
+    bits = (bits + 0x7FFF + ((bits >> 16) & 1)) >> 16
+    return (bits.astype(np.uint32) << 16).view(np.float32)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+@pytest.mark.parametrize("name,A", MATS[:7], ids=[m[0] for m in MATS[:7]])
+def test_spmv_low_precision(dtype, name, A):
+    M64 = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    x = P.uniform(9, A.ncols, -1, 1).astype(np.float32)
+    v32 = A.val.astype(np.float32) if dtype == "f32" else _bf16_round(A.val)
+    want = O.spmv_f32(_ocsr(A), v32, x)
+    for M in (S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols, dtype=dtype), M64.like(dtype)):
+        y = M.mult(_dev(x, torch.float32)).cpu().numpy()
+        assert np.array_equal(y, want)
+
+
+def test_spmv_full_size_27pt_ghost_plane():
+    """One rank's 160^3 share of the 27-point stencil with its ghost plane
+    (109,443,836 stored entries), bit-exact against the oracle."""
+    A = P.stencil(27, 160, 160, 160, ghost_planes=1)
+    assert A.nnz == 109_443_836
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    x = P.uniform(21, A.ncols, -1, 1)
+    y = M.mult(_dev(x)).cpu().numpy()
+    assert np.array_equal(y, O.spmv(_ocsr(A), x, threads=8))
+
+
+def test_spmv_empty_matrix():
+    M = S.Matrix(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0), ncols=0)
+    y = M.mult(torch.zeros(0, dtype=torch.float64, device=DEV))
+    assert y.numel() == 0
+
+
+# ---------------------------------------------------------------- L2GMap
+def test_update_pack_unpack():
+    kw = P.slab_decomposition(9, 8, 7)
+    m = S.L2GMap(**kw)
+    n = m.local_size() + m.num_ghosts()
+    v = P.uniform(1, n, -1, 1)
+    halo = P.uniform(2, m.num_ghosts(), -5, 5)
+    m.set_halo(halo)
+    vd = _dev(v)
+    send = m.update(vd, send_out=True)
+    vo = v.copy()
+    send_o = O.l2g_update(m.local_size(), m.num_ghosts(), m.plan()["indexbuf"], vo, halo)
+    assert np.array_equal(vd.cpu().numpy(), vo)
+    assert np.array_equal(send, send_o)
+    # halo reset to zeros
+    m.set_halo(None)
+    m.update(vd)
+    assert np.all(vd.cpu().numpy()[m.local_size():] == 0.0)
+
+
+def test_update_float32():
+    kw = P.slab_decomposition(5, 4, 6)
+    m = S.L2GMap(**kw)
+    n = m.local_size() + m.num_ghosts()
+    v = P.uniform(3, n, -1, 1).astype(np.float32)
+    halo = P.uniform(4, m.num_ghosts(), -1, 1).astype(np.float32)
+    m.set_halo(halo, dtype=S.F32)
+    vd = _dev(v, torch.float32)
+    send = m.update(vd, send_out=True)
+    assert np.array_equal(send, v[m.plan()["indexbuf"]])
+    assert np.array_equal(vd.cpu().numpy()[m.local_size():], halo)
+
+
+def test_reverse_update_repeated_indices():
+    # rank 1 of 3 in the "scatter" decomposition: indexbuf repeats indices
+    import json, os
+    gold = [json.load(open(os.path.join(os.path.dirname(__file__), "golden", "l2gmap",
+                                        f"scatter_3_{r}.json"))) for r in range(3)]
+    r = 1
+    g = gold[r]
+    lo, hi = g["ranges"][r], g["ranges"][r + 1]
+    remote = {q: [x for x in gold[q]["ghosts"] if lo <= x < hi] for q in range(3) if q != r}
+    m = S.L2GMap(g["local_size"], g["ghosts"], local_sizes=[x["local_size"] for x in gold], rank=r,
+                 remote_ghosts=remote)
+    idx = m.plan()["indexbuf"]
+    assert len(set(idx)) < len(idx)
+    n = m.local_size() + m.num_ghosts()
+    v = P.uniform(5, n, -1, 1)
+    inc = P.uniform(6, len(idx), -1, 1)
+    vd = _dev(v)
+    out = m.reverse_update(vd, inc)
+    vo = v.copy()
+    out_o = O.l2g_reverse_update(m.local_size(), m.num_ghosts(), idx, vo, inc)
+    assert np.array_equal(out, out_o)
+    assert np.array_equal(vd.cpu().numpy(), vo)
+
+
+# ---------------------------------------------------------------- SDDMM
+def test_sddmm():
+    A = P.stencil(5, 17, 19)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    u = P.uniform(1, A.nrows, -1, 1)
+    v = P.uniform(2, A.ncols, -1, 1)
+    g = S.sddmm(M, _dev(u), _dev(v), alpha=-1.0).cpu().numpy()
+    rows = np.repeat(np.arange(A.nrows), np.diff(A.rowptr))
+    assert np.array_equal(g, (-u[rows]) * v[A.colidx])
