@@ -156,6 +156,12 @@ int spmvx_matrix_info(spmvx_matrix A, int32_t* dtype, int64_t* nrows, int64_t* n
 /* Device pointers of the stored arrays (values in the stored dtype). */
 int spmvx_matrix_arrays(spmvx_matrix A, const int32_t** rowptr_dev,
                         const int32_t** colidx_dev, const void** values_dev);
+/* Selects the SpMV kernel variant of A (staging chunk, loads in flight,
+ * streaming loads; 0 <= variant < spmvx_num_kernels(), -1 = default).  All
+ * variants give identical results; used by tuning.  No reference
+ * counterpart (the inspector of Matrix.cpp:43 picks its own kernel). */
+int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant);
+int spmvx_num_kernels(void);
 /* Overwrite the values from nnz doubles on the device (pattern unchanged). */
 int spmvx_matrix_set_values(spmvx_matrix A, const double* values_dev, spmvx_stream stream);
 
@@ -202,21 +208,61 @@ typedef struct
   int32_t inner_hist[64]; /* inner iterations of step j                       */
 } spmvx_ir_stats;
 
+/* spmvx_ir_cg flags */
+#define SPMVX_IR_CARRY_DIRECTION 1
+
 /*
  * Solves A64 x = b to ||b - A64 x|| / ||b|| <= rtol.  Each outer step forms
  * r = b - A64 x (FP64 SpMV) and ||r|| in FP64 on the device, runs CG in FP32
  * on Alow d = r / ||r|| (Alow: SPMVX_F32 or SPMVX_BF16 copy with the same
  * pattern; FP32 vectors and accumulation; stop at relative residual
  * inner_rtol or inner_max_its) and updates x += ||r|| d in FP64.
+ * flags = 0: every inner solve restarts from d = 0, p = r (classic
+ * refinement).  SPMVX_IR_CARRY_DIRECTION: the inner solve keeps its previous
+ * search direction, rescaled to the new normalisation, and only replaces the
+ * residual by the FP64 one (residual replacement / "reliable updates"): the
+ * Krylov information survives the outer steps.
  * x_dev[ncols of A64]; ghost columns are treated as zero.
  */
 int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b_dev, double* x_dev,
                 int32_t max_outer, double rtol, int32_t inner_max_its, double inner_rtol,
-                spmvx_ir_stats* stats, spmvx_stream stream);
+                int32_t flags, spmvx_ir_stats* stats, spmvx_stream stream);
 
 /* FP32 CG on an SPMVX_F32 / SPMVX_BF16 matrix (the inner solver alone). */
 int spmvx_cg_f32(spmvx_matrix A, const float* b_dev, float* x_dev, int32_t max_its,
                  double rtol, int32_t* its, spmvx_stream stream);
+
+/* ===================================================================== */
+/* Extension 2: batched persistent CG (no reference counterpart; each     */
+/* system runs spmv::cg's loop, cg.cpp:50-80, in FP32)                     */
+/* ===================================================================== */
+/*
+ * nsys independent SPD systems given as the diagonal blocks of one
+ * block-diagonal CSR matrix: system s owns rows and columns
+ * [sys_row[s], sys_row[s+1]); rowptr[sys_row[nsys]+1] and colidx (global
+ * column indices, each inside its own system) and FP32 values.  nrhs (1, 2
+ * or 4) right-hand sides per system.  Vectors are [row][nrhs] row-major.
+ */
+int spmvx_batch_create(int32_t nsys, const int32_t* sys_row_host, const int32_t* rowptr_host,
+                       const int32_t* colidx_host, const float* values_host, int32_t nrhs,
+                       spmvx_batch* out);
+int spmvx_batch_destroy(spmvx_batch B);
+/* Launch configuration: workgroup size (256, 512, 1024) and lanes per row
+ * (1, 2, 4, 8, 16; 0 = chosen per system from its mean row length).  Set by
+ * the autotuner (spmv_amd/autotune.py). */
+int spmvx_batch_set_config(spmvx_batch B, int32_t block, int32_t lanes_per_row);
+/*
+ * Solves every system and right-hand side with CG from x = 0 until
+ * ||r_k||^2 / ||b||^2 < rtol^2 or max_its, in one persistent kernel launch
+ * (one workgroup per system, systems taken largest first from a device work
+ * queue), replayed from a hipGraph captured on the first call with these
+ * arguments.  its_host (may be NULL): nsys*nrhs iteration counts; when given
+ * the stream is synchronised.
+ */
+int spmvx_batch_solve(spmvx_batch B, const float* b_dev, float* x_dev, int32_t max_its,
+                      double rtol, int32_t* its_host, spmvx_stream stream);
+int spmvx_batch_info(spmvx_batch B, int32_t* nsys, int64_t* nrows, int64_t* nnz, int32_t* nrhs,
+                     int32_t* block, int32_t* lanes_per_row, int32_t* grid);
 
 /* ===================================================================== */
 /* Extension 3: adjoint helpers (no reference counterpart)                */

@@ -23,7 +23,7 @@ import os
 
 import numpy as np
 
-__all__ = ["L2GMap", "Matrix", "cg", "cg_f32", "ir_cg", "sddmm", "copy_f64", "SpmvError",
+__all__ = ["L2GMap", "Matrix", "Batch", "cg", "cg_f32", "ir_cg", "sddmm", "copy_f64", "SpmvError",
            "F64", "F32", "BF16", "lib", "library_path"]
 
 F64, F32, BF16 = 0, 1, 2
@@ -80,13 +80,20 @@ def lib():
         "spmvx_matrix_info": ([P, P, P, P, P, P], ctypes.c_int),
         "spmvx_matrix_arrays": ([P, P, P, P], ctypes.c_int),
         "spmvx_matrix_set_values": ([P, P, P], ctypes.c_int),
+        "spmvx_matrix_set_kernel": ([P, i32], ctypes.c_int),
+        "spmvx_num_kernels": ([], ctypes.c_int),
         "spmvx_spmv": ([P, P, P, P], ctypes.c_int),
         "spmvx_spmv_t": ([P, P, P, P], ctypes.c_int),
         "spmvx_cg": ([P, P, P, P, i32, f64, P, P, P], ctypes.c_int),
         "spmvx_cg_f32": ([P, P, P, i32, f64, P, P], ctypes.c_int),
-        "spmvx_ir_cg": ([P, P, P, P, i32, f64, i32, f64, P, P], ctypes.c_int),
+        "spmvx_ir_cg": ([P, P, P, P, i32, f64, i32, f64, i32, P, P], ctypes.c_int),
         "spmvx_sddmm": ([P, f64, P, P, P, P], ctypes.c_int),
         "spmvx_copy_f64": ([P, P, i64, P], ctypes.c_int),
+        "spmvx_batch_create": ([i32, P, P, P, P, i32, P], ctypes.c_int),
+        "spmvx_batch_destroy": ([P], ctypes.c_int),
+        "spmvx_batch_set_config": ([P, i32, i32], ctypes.c_int),
+        "spmvx_batch_solve": ([P, P, P, i32, f64, P, P], ctypes.c_int),
+        "spmvx_batch_info": ([P, P, P, P, P, P, P, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -346,6 +353,10 @@ class Matrix:
         _check(lib().spmvx_spmv_t(self._h, _dptr(x), _dptr(out), _stream(stream)))
         return out
 
+    def set_kernel(self, variant):
+        """SpMV kernel variant (-1 = default); results are identical."""
+        _check(lib().spmvx_matrix_set_kernel(self._h, int(variant)))
+
     def set_values(self, values_f64, stream=None):
         _check(lib().spmvx_matrix_set_values(self._h, _dptr(values_f64), _stream(stream)))
 
@@ -393,8 +404,10 @@ def cg_f32(A, b, max_its, rtol, stream=None):
 
 
 def ir_cg(A64, Alow, b, rtol=1e-10, inner_rtol=1e-4, max_outer=32, inner_max_its=100000,
-          stream=None):
-    """Mixed-precision iterative refinement: returns (x, stats dict)."""
+          carry_direction=False, stream=None):
+    """Mixed-precision iterative refinement: returns (x, stats dict).
+    carry_direction keeps the inner CG's search direction across outer steps
+    (residual replacement) instead of restarting it."""
     import torch
     if b.numel() != A64.nrows or b.dtype != torch.float64:
         raise SpmvError("b must be float64 with A.nrows entries")
@@ -402,7 +415,7 @@ def ir_cg(A64, Alow, b, rtol=1e-10, inner_rtol=1e-4, max_outer=32, inner_max_its
     st = IRStats()
     _check(lib().spmvx_ir_cg(A64.handle, Alow.handle, _dptr(b), _dptr(x), int(max_outer),
                              float(rtol), int(inner_max_its), float(inner_rtol),
-                             ctypes.byref(st), _stream(stream)))
+                             1 if carry_direction else 0, ctypes.byref(st), _stream(stream)))
     return x, st.as_dict()
 
 
@@ -424,3 +437,63 @@ def copy_f64(dst, src, stream=None):
     if dst.numel() < src.numel():
         raise SpmvError("dst is shorter than src")
     _check(lib().spmvx_copy_f64(_dptr(dst), _dptr(src), src.numel(), _stream(stream)))
+
+
+# =========================================================================
+class Batch:
+    """Many independent SPD systems solved by one persistent CG launch.
+
+    ``systems`` is a list of host CSRs (objects with rowptr, colidx, val,
+    nrows); they are stacked into one block-diagonal matrix.  Vectors are
+    torch float32 tensors of shape (total_rows, nrhs).
+    """
+
+    def __init__(self, systems, nrhs=4):
+        sys_row = np.zeros(len(systems) + 1, np.int64)
+        sys_row[1:] = np.cumsum([A.nrows for A in systems])
+        nnz = np.zeros(len(systems) + 1, np.int64)
+        nnz[1:] = np.cumsum([int(A.rowptr[-1]) for A in systems])
+        if sys_row[-1] >= 2 ** 31 or nnz[-1] >= 2 ** 31:
+            raise SpmvError("batch too large for 32-bit indices")
+        rowptr = np.concatenate([np.asarray(A.rowptr[:-1], np.int64) + nnz[s]
+                                 for s, A in enumerate(systems)] + [nnz[-1:]]).astype(np.int32)
+        colidx = np.concatenate([np.asarray(A.colidx, np.int64) + sys_row[s]
+                                 for s, A in enumerate(systems)]).astype(np.int32)
+        vals = np.concatenate([np.asarray(A.val, np.float32) for A in systems])
+        self.sys_row = sys_row.astype(np.int32)
+        self.nsys, self.nrhs = len(systems), int(nrhs)
+        self.nrows, self.nnz = int(sys_row[-1]), int(nnz[-1])
+        h = ctypes.c_void_p()
+        _check(lib().spmvx_batch_create(self.nsys, _np_ptr(self.sys_row), _np_ptr(rowptr),
+                                        _np_ptr(colidx), _np_ptr(vals), self.nrhs, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.spmvx_batch_destroy(self._h)
+            self._h = None
+
+    def set_config(self, block, lanes_per_row):
+        _check(lib().spmvx_batch_set_config(self._h, int(block), int(lanes_per_row)))
+
+    def config(self):
+        v = [ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32(),
+             ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()]
+        _check(lib().spmvx_batch_info(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(block=v[4].value, lanes_per_row=v[5].value, grid=v[6].value)
+
+    def solve(self, b, x=None, max_its=10000, rtol=1e-6, iterations=True, stream=None):
+        """Returns x (and the (nsys, nrhs) iteration counts when iterations)."""
+        import torch
+        if b.dtype != torch.float32 or b.shape != (self.nrows, self.nrhs) or not b.is_contiguous():
+            raise SpmvError("b must be a contiguous float32 tensor of shape (rows, nrhs)")
+        if x is None:
+            x = torch.empty_like(b)
+        elif x.shape != b.shape or x.dtype != b.dtype:
+            raise SpmvError("x must match b")
+        its = np.zeros(self.nsys * self.nrhs, np.int32) if iterations else None
+        _check(lib().spmvx_batch_solve(self._h, _dptr(b), _dptr(x), int(max_its), float(rtol),
+                                       _np_ptr(its), _stream(stream)))
+        if iterations:
+            return x, its.reshape(self.nsys, self.nrhs)
+        return x

@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libspmv_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPMV_AMD_ARCH", "gfx950")
 
-SOURCES = ["capi.cpp", "matrix.hip", "l2gmap.hip", "cg.hip"]
+SOURCES = ["capi.cpp", "matrix.hip", "l2gmap.hip", "cg.hip", "batch.hip"]
 
 # -ffp-contract=off: products and sums are rounded separately (the FP64
 # kernels are compared bit for bit with the reference's Eigen arithmetic).

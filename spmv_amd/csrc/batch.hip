@@ -1,0 +1,510 @@
+// Batched CG: many independent SPD systems (the diagonal blocks of one
+// block-diagonal CSR matrix) solved by ONE persistent kernel launch, captured
+// in a hipGraph.  No reference counterpart; each system runs the loop of
+// spmv::cg (spmv/cg.cpp:50-80) in FP32 with NRHS right-hand sides at once.
+//
+// Design (gfx950):
+//   * one workgroup solves one whole system (all right-hand sides); all of
+//     its synchronisation is workgroup-local (__syncthreads), so workgroups
+//     never wait on each other and the grid always drains;
+//   * systems are pulled from a device work queue (atomic counter) in
+//     decreasing size order, so the largest start first (LPT schedule);
+//   * vectors are stored row-major [row][NRHS]: one 16-byte gather of p[c]
+//     serves all four right-hand sides, and the matrix is read once per
+//     iteration for all of them;
+//   * G lanes cooperate on a row (rows per wavefront = 64/G), chosen per
+//     system; G and the block size are picked by the autotuner.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "handles.hpp"
+
+namespace spmvx
+{
+namespace
+{
+
+constexpr int kMaxBatchBS = 1024;
+
+template <int NRHS>
+struct VecN
+{
+  float v[NRHS];
+};
+
+template <int NRHS>
+__device__ __forceinline__ VecN<NRHS> ldn(const float* p)
+{
+  VecN<NRHS> r;
+  if constexpr (NRHS == 4)
+  {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    r.v[0] = q.x;
+    r.v[1] = q.y;
+    r.v[2] = q.z;
+    r.v[3] = q.w;
+  }
+  else if constexpr (NRHS == 2)
+  {
+    const float2 q = *reinterpret_cast<const float2*>(p);
+    r.v[0] = q.x;
+    r.v[1] = q.y;
+  }
+  else
+    r.v[0] = *p;
+  return r;
+}
+
+template <int NRHS>
+__device__ __forceinline__ void stn(float* p, const VecN<NRHS>& r)
+{
+  if constexpr (NRHS == 4)
+    *reinterpret_cast<float4*>(p) = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
+  else if constexpr (NRHS == 2)
+    *reinterpret_cast<float2*>(p) = make_float2(r.v[0], r.v[1]);
+  else
+    *p = r.v[0];
+}
+
+// Sum over the block of NRHS values per thread; result in all threads.
+template <int NRHS>
+__device__ __forceinline__ VecN<NRHS> block_sum_n(VecN<NRHS> a, float (*red)[16])
+{
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NRHS; ++j)
+    a.v[j] = wave_sum(a.v[j]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < NRHS; ++j)
+      red[j][wid] = a.v[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NRHS; ++j)
+    a.v[j] = wave_sum(lane < nw ? red[j][lane] : 0.0f);
+  return a;
+}
+
+// y = A p over rows [r0, r1) with G lanes per row; returns this thread's
+// share of p.y (non-zero only in group leaders).
+template <int NRHS, int G>
+__device__ __forceinline__ VecN<NRHS> batch_spmv(const int32_t* __restrict__ rowptr,
+                                                 const int32_t* __restrict__ colidx,
+                                                 const float* __restrict__ val,
+                                                 const float* __restrict__ p,
+                                                 float* __restrict__ y, int32_t r0, int32_t r1)
+{
+  VecN<NRHS> pdy{};
+  const int gl = threadIdx.x % G;
+  const int rows_per_pass = blockDim.x / G;
+  for (int32_t row = r0 + static_cast<int32_t>(threadIdx.x / G); row < r1; row += rows_per_pass)
+  {
+    VecN<NRHS> acc{};
+    const int32_t e1 = rowptr[row + 1];
+    for (int32_t e = rowptr[row] + gl; e < e1; e += G)
+    {
+      const float a = val[e];
+      const VecN<NRHS> pc = ldn<NRHS>(p + static_cast<int64_t>(colidx[e]) * NRHS);
+#pragma unroll
+      for (int j = 0; j < NRHS; ++j)
+        acc.v[j] += a * pc.v[j];
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int j = 0; j < NRHS; ++j)
+        acc.v[j] += __shfl_xor(acc.v[j], off, 64);
+    if (gl == 0)
+    {
+      stn<NRHS>(y + static_cast<int64_t>(row) * NRHS, acc);
+      const VecN<NRHS> pr = ldn<NRHS>(p + static_cast<int64_t>(row) * NRHS);
+#pragma unroll
+      for (int j = 0; j < NRHS; ++j)
+        pdy.v[j] += pr.v[j] * acc.v[j];
+    }
+  }
+  return pdy;
+}
+
+template <int NRHS>
+__global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+    const float* __restrict__ val, const int32_t* __restrict__ sys_row,
+    const int32_t* __restrict__ order, const int8_t* __restrict__ gsel, int32_t nsys,
+    const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
+    float* __restrict__ p, float* __restrict__ y, int32_t max_its, float rtol2,
+    int32_t* __restrict__ its_out, unsigned* queue)
+{
+  __shared__ int32_t job;
+  __shared__ float red[NRHS][16];
+  for (;;)
+  {
+    __syncthreads();
+    if (threadIdx.x == 0)
+      job = static_cast<int32_t>(atomicAdd(queue, 1u));
+    __syncthreads();
+    const int32_t jb = job;
+    if (jb >= nsys)
+      break;
+    const int32_t s = order[jb];
+    const int32_t r0 = sys_row[s], r1 = sys_row[s + 1];
+    const int g = gsel[s];
+
+    // x = 0, r = b, p = b, rr0 = ||b||^2 (cg.cpp:31-44)
+    VecN<NRHS> rr{};
+    for (int32_t i = r0 + static_cast<int32_t>(threadIdx.x); i < r1; i += blockDim.x)
+    {
+      const int64_t o = static_cast<int64_t>(i) * NRHS;
+      const VecN<NRHS> bi = ldn<NRHS>(b + o);
+      stn<NRHS>(x + o, VecN<NRHS>{});
+      stn<NRHS>(r + o, bi);
+      stn<NRHS>(p + o, bi);
+#pragma unroll
+      for (int j = 0; j < NRHS; ++j)
+        rr.v[j] += bi.v[j] * bi.v[j];
+    }
+    rr = block_sum_n<NRHS>(rr, red);
+    const VecN<NRHS> rr0 = rr;
+    bool active[NRHS];
+    int32_t its[NRHS];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < NRHS; ++j)
+    {
+      active[j] = max_its > 0;
+      its[j] = 0;
+      any = any || active[j];
+    }
+    for (int32_t k = 1; k <= max_its && any; ++k)
+    {
+      __syncthreads(); // p complete
+      VecN<NRHS> pdy;
+      switch (g)
+      {
+      case 1: pdy = batch_spmv<NRHS, 1>(rowptr, colidx, val, p, y, r0, r1); break;
+      case 2: pdy = batch_spmv<NRHS, 2>(rowptr, colidx, val, p, y, r0, r1); break;
+      case 4: pdy = batch_spmv<NRHS, 4>(rowptr, colidx, val, p, y, r0, r1); break;
+      case 8: pdy = batch_spmv<NRHS, 8>(rowptr, colidx, val, p, y, r0, r1); break;
+      default: pdy = batch_spmv<NRHS, 16>(rowptr, colidx, val, p, y, r0, r1); break;
+      }
+      pdy = block_sum_n<NRHS>(pdy, red); // also orders y writes before the reads below
+      float alpha[NRHS];
+#pragma unroll
+      for (int j = 0; j < NRHS; ++j)
+        alpha[j] = active[j] ? rr.v[j] / pdy.v[j] : 0.0f;
+      VecN<NRHS> rn{};
+      for (int32_t i = r0 + static_cast<int32_t>(threadIdx.x); i < r1; i += blockDim.x)
+      {
+        const int64_t o = static_cast<int64_t>(i) * NRHS;
+        VecN<NRHS> ri = ldn<NRHS>(r + o), xi = ldn<NRHS>(x + o);
+        const VecN<NRHS> yi = ldn<NRHS>(y + o), pi = ldn<NRHS>(p + o);
+#pragma unroll
+        for (int j = 0; j < NRHS; ++j)
+        {
+          xi.v[j] = xi.v[j] + alpha[j] * pi.v[j];
+          ri.v[j] = ri.v[j] - alpha[j] * yi.v[j];
+          rn.v[j] += ri.v[j] * ri.v[j];
+        }
+        stn<NRHS>(x + o, xi);
+        stn<NRHS>(r + o, ri);
+      }
+      rn = block_sum_n<NRHS>(rn, red);
+      float beta[NRHS];
+      any = false;
+#pragma unroll
+      for (int j = 0; j < NRHS; ++j)
+      {
+        beta[j] = active[j] ? rn.v[j] / rr.v[j] : 0.0f;
+        if (active[j])
+        {
+          its[j] = k;
+          rr.v[j] = rn.v[j];
+          if (rn.v[j] / rr0.v[j] < rtol2 || !(rn.v[j] == rn.v[j]))
+            active[j] = false;
+        }
+        any = any || active[j];
+      }
+      if (!any)
+        break;
+      for (int32_t i = r0 + static_cast<int32_t>(threadIdx.x); i < r1; i += blockDim.x)
+      {
+        const int64_t o = static_cast<int64_t>(i) * NRHS;
+        const VecN<NRHS> ri = ldn<NRHS>(r + o);
+        VecN<NRHS> pi = ldn<NRHS>(p + o);
+#pragma unroll
+        for (int j = 0; j < NRHS; ++j)
+          pi.v[j] = pi.v[j] * beta[j] + ri.v[j];
+        stn<NRHS>(p + o, pi);
+      }
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int j = 0; j < NRHS; ++j)
+        its_out[static_cast<int64_t>(s) * NRHS + j] = its[j];
+  }
+}
+
+__global__ void reset_counter(unsigned* c) { *c = 0u; }
+
+int pow2_ceil(int v)
+{
+  int g = 1;
+  while (g < v)
+    g <<= 1;
+  return g;
+}
+
+} // namespace
+} // namespace spmvx
+
+using namespace spmvx;
+
+struct spmvx_batch_s
+{
+  int32_t nsys = 0, nrhs = 0;
+  int64_t nrows = 0, nnz = 0;
+  std::vector<int32_t> h_sys_row, h_rowptr;
+  DevBuf rowptr, colidx, val, sys_row, order, gsel, scratch, its, queue;
+  int32_t block = 1024;
+  int32_t g_policy = 0; // 0 = per system from the mean row length, else fixed G
+  int grid = 0;
+  // captured graph and the arguments it was captured with
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  const void* cap_b = nullptr;
+  void* cap_x = nullptr;
+  int32_t cap_its = -1, cap_block = -1, cap_policy = -1;
+  float cap_rtol2 = -1.0f;
+  // the batch's own stream (graphs cannot be captured on the null stream),
+  // ordered after / before the caller's stream with events
+  hipStream_t own = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  ~spmvx_batch_s()
+  {
+    if (exec)
+      (void)hipGraphExecDestroy(exec);
+    if (graph)
+      (void)hipGraphDestroy(graph);
+    if (ev_in)
+      (void)hipEventDestroy(ev_in);
+    if (ev_out)
+      (void)hipEventDestroy(ev_out);
+    if (own)
+      (void)hipStreamDestroy(own);
+  }
+};
+
+namespace
+{
+
+void apply_policy(spmvx_batch_s* B)
+{
+  std::vector<int8_t> g(B->nsys);
+  for (int s = 0; s < B->nsys; ++s)
+  {
+    if (B->g_policy > 0)
+      g[s] = static_cast<int8_t>(B->g_policy);
+    else
+    {
+      const int64_t rows = B->h_sys_row[s + 1] - B->h_sys_row[s];
+      const int64_t nz = B->h_rowptr[B->h_sys_row[s + 1]] - B->h_rowptr[B->h_sys_row[s]];
+      const int mean = rows > 0 ? static_cast<int>((nz + rows - 1) / rows) : 1;
+      g[s] = static_cast<int8_t>(std::min(16, pow2_ceil(std::max(1, mean / 4))));
+    }
+  }
+  SPMVX_HIP(hipMemcpy(B->gsel.ptr, g.data(), g.size(), hipMemcpyHostToDevice));
+}
+
+void launch_batch(spmvx_batch_s* B, const float* b, float* x, int32_t max_its, float rtol2,
+                  hipStream_t s)
+{
+  const int64_t vn = B->nrows * B->nrhs;
+  float* r = B->scratch.as<float>();
+  float* p = r + vn;
+  float* y = p + vn;
+  reset_counter<<<1, 1, 0, s>>>(B->queue.as<unsigned>());
+#define SPMVX_BATCH(NR)                                                                        \
+  batch_cg_kernel<NR><<<B->grid, B->block, 0, s>>>(                                           \
+      B->rowptr.as<int32_t>(), B->colidx.as<int32_t>(), B->val.as<float>(),                    \
+      B->sys_row.as<int32_t>(), B->order.as<int32_t>(), B->gsel.as<int8_t>(), B->nsys, b, x,   \
+      r, p, y, max_its, rtol2, B->its.as<int32_t>(), B->queue.as<unsigned>())
+  if (B->nrhs == 4)
+    SPMVX_BATCH(4);
+  else if (B->nrhs == 2)
+    SPMVX_BATCH(2);
+  else
+    SPMVX_BATCH(1);
+#undef SPMVX_BATCH
+}
+
+} // namespace
+
+extern "C" {
+
+int spmvx_batch_create(int32_t nsys, const int32_t* sys_row, const int32_t* rowptr,
+                       const int32_t* colidx, const float* values, int32_t nrhs,
+                       spmvx_batch* out)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(out != nullptr, "out is NULL");
+    *out = nullptr;
+    SPMVX_REQUIRE(nsys >= 0 && sys_row != nullptr && rowptr != nullptr, "invalid batch");
+    SPMVX_REQUIRE(nrhs == 1 || nrhs == 2 || nrhs == 4, "nrhs must be 1, 2 or 4");
+    SPMVX_REQUIRE(sys_row[0] == 0, "sys_row[0] must be 0");
+    for (int s = 0; s < nsys; ++s)
+      SPMVX_REQUIRE(sys_row[s + 1] >= sys_row[s], "sys_row must be non-decreasing");
+    const int64_t nrows = sys_row[nsys];
+    const int64_t nnz = rowptr[nrows];
+    SPMVX_REQUIRE(rowptr[0] == 0, "rowptr[0] must be 0");
+    for (int64_t i = 0; i < nrows; ++i)
+      SPMVX_REQUIRE(rowptr[i + 1] >= rowptr[i], "rowptr must be non-decreasing");
+    SPMVX_REQUIRE(nnz == 0 || (colidx && values), "NULL arrays");
+    for (int s = 0; s < nsys; ++s)
+      for (int64_t i = sys_row[s]; i < sys_row[s + 1]; ++i)
+        for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+          SPMVX_REQUIRE(colidx[k] >= sys_row[s] && colidx[k] < sys_row[s + 1],
+                        "column outside its system (matrix must be block diagonal)");
+    auto B = std::make_unique<spmvx_batch_s>();
+    B->nsys = nsys;
+    B->nrhs = nrhs;
+    B->nrows = nrows;
+    B->nnz = nnz;
+    B->h_sys_row.assign(sys_row, sys_row + nsys + 1);
+    B->h_rowptr.assign(rowptr, rowptr + nrows + 1);
+    B->rowptr.ensure(4 * (nrows + 1));
+    SPMVX_HIP(hipMemcpy(B->rowptr.ptr, rowptr, 4 * (nrows + 1), hipMemcpyHostToDevice));
+    B->colidx.ensure(4 * std::max<int64_t>(nnz, 1));
+    B->val.ensure(4 * std::max<int64_t>(nnz, 1));
+    if (nnz)
+    {
+      SPMVX_HIP(hipMemcpy(B->colidx.ptr, colidx, 4 * nnz, hipMemcpyHostToDevice));
+      SPMVX_HIP(hipMemcpy(B->val.ptr, values, 4 * nnz, hipMemcpyHostToDevice));
+    }
+    B->sys_row.ensure(4 * (nsys + 1));
+    SPMVX_HIP(hipMemcpy(B->sys_row.ptr, sys_row, 4 * (nsys + 1), hipMemcpyHostToDevice));
+    // LPT order: largest (rows + entries) first
+    std::vector<int32_t> order(nsys);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
+      const int64_t wa = (sys_row[a + 1] - sys_row[a]) + (rowptr[sys_row[a + 1]] - rowptr[sys_row[a]]);
+      const int64_t wc = (sys_row[c + 1] - sys_row[c]) + (rowptr[sys_row[c + 1]] - rowptr[sys_row[c]]);
+      return wa > wc;
+    });
+    B->order.ensure(4 * std::max(nsys, 1));
+    if (nsys)
+      SPMVX_HIP(hipMemcpy(B->order.ptr, order.data(), 4 * nsys, hipMemcpyHostToDevice));
+    B->gsel.ensure(std::max(nsys, 1));
+    B->scratch.ensure(sizeof(float) * 3 * static_cast<size_t>(std::max<int64_t>(nrows, 1)) * nrhs);
+    B->its.ensure(4 * static_cast<size_t>(std::max(nsys, 1)) * nrhs);
+    B->queue.ensure(256);
+    apply_policy(B.get());
+    *out = B.release();
+  });
+}
+
+int spmvx_batch_destroy(spmvx_batch B)
+{
+  return guarded([&] { delete B; });
+}
+
+int spmvx_batch_set_config(spmvx_batch B, int32_t block, int32_t lanes_per_row)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(B != nullptr, "NULL batch");
+    SPMVX_REQUIRE(block == 256 || block == 512 || block == 1024, "block must be 256, 512 or 1024");
+    SPMVX_REQUIRE(lanes_per_row == 0 || lanes_per_row == 1 || lanes_per_row == 2
+                      || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16,
+                  "lanes_per_row must be 0 (per system), 1, 2, 4, 8 or 16");
+    B->block = block;
+    B->g_policy = lanes_per_row;
+    apply_policy(B);
+  });
+}
+
+int spmvx_batch_solve(spmvx_batch B, const float* b, float* x, int32_t max_its, double rtol,
+                      int32_t* its_host, spmvx_stream stream)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(B != nullptr, "NULL batch");
+    SPMVX_REQUIRE(max_its >= 0, "max_its must be >= 0");
+    SPMVX_REQUIRE(B->nrows == 0 || (b && x), "NULL vector");
+    hipStream_t s = as_stream(stream);
+    if (B->nsys == 0)
+      return;
+    // persistent grid: every workgroup resident at once
+    int per_cu = 0;
+    if (B->nrhs == 4)
+      SPMVX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_cg_kernel<4>, B->block, 0));
+    else if (B->nrhs == 2)
+      SPMVX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_cg_kernel<2>, B->block, 0));
+    else
+      SPMVX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_cg_kernel<1>, B->block, 0));
+    int dev = 0, ncu = 0;
+    SPMVX_HIP(hipGetDevice(&dev));
+    SPMVX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    B->grid = std::max(1, std::min(B->nsys, std::max(1, per_cu) * ncu));
+    const float rtol2 = static_cast<float>(rtol * rtol);
+    if (!B->own)
+    {
+      SPMVX_HIP(hipStreamCreateWithFlags(&B->own, hipStreamNonBlocking));
+      SPMVX_HIP(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming));
+      SPMVX_HIP(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming));
+    }
+    const bool fresh = !B->exec || B->cap_b != b || B->cap_x != x || B->cap_its != max_its
+                       || B->cap_rtol2 != rtol2 || B->cap_block != B->block
+                       || B->cap_policy != B->g_policy;
+    if (fresh)
+    {
+      if (B->exec)
+        SPMVX_HIP(hipGraphExecDestroy(B->exec));
+      if (B->graph)
+        SPMVX_HIP(hipGraphDestroy(B->graph));
+      B->exec = nullptr;
+      B->graph = nullptr;
+      SPMVX_HIP(hipStreamBeginCapture(B->own, hipStreamCaptureModeThreadLocal));
+      launch_batch(B, b, x, max_its, rtol2, B->own);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(B->own, &g);
+      SPMVX_HIP(e);
+      B->graph = g;
+      SPMVX_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+      B->cap_b = b;
+      B->cap_x = x;
+      B->cap_its = max_its;
+      B->cap_rtol2 = rtol2;
+      B->cap_block = B->block;
+      B->cap_policy = B->g_policy;
+    }
+    SPMVX_HIP(hipEventRecord(B->ev_in, s));
+    SPMVX_HIP(hipStreamWaitEvent(B->own, B->ev_in, 0));
+    SPMVX_HIP(hipGraphLaunch(B->exec, B->own));
+    SPMVX_HIP(hipEventRecord(B->ev_out, B->own));
+    SPMVX_HIP(hipStreamWaitEvent(s, B->ev_out, 0));
+    if (its_host)
+    {
+      SPMVX_HIP(hipMemcpyAsync(its_host, B->its.ptr, 4 * static_cast<size_t>(B->nsys) * B->nrhs,
+                               hipMemcpyDeviceToHost, s));
+      SPMVX_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
+int spmvx_batch_info(spmvx_batch B, int32_t* nsys, int64_t* nrows, int64_t* nnz, int32_t* nrhs,
+                     int32_t* block, int32_t* lanes_per_row, int32_t* grid)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(B != nullptr, "NULL batch");
+    if (nsys) *nsys = B->nsys;
+    if (nrows) *nrows = B->nrows;
+    if (nnz) *nnz = B->nnz;
+    if (nrhs) *nrhs = B->nrhs;
+    if (block) *block = B->block;
+    if (lanes_per_row) *lanes_per_row = B->g_policy;
+    if (grid) *grid = B->grid;
+  });
+}
+
+} // extern "C"

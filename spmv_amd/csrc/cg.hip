@@ -19,24 +19,22 @@ namespace
 {
 
 template <typename X, typename Acc>
-void ensure_ws(CgWorkspace& w, const Pattern& P, int64_t ncols)
+void ensure_ws(CgWorkspace& w, const Pattern& P, int64_t ncols, int64_t nblocks)
 {
   const int64_t n = P.nrows;
-  if (w.mem.ptr && w.n == n && w.ncols == ncols && w.nblocks == P.nblocks && w.elem == sizeof(X))
+  if (w.mem.ptr && w.n == n && w.ncols == ncols && w.nblocks == nblocks && w.elem == sizeof(X))
     return;
   const int vgrid = vec_grid(n > ncols ? n : ncols);
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   const size_t vec_n = al(sizeof(X) * static_cast<size_t>(n > 0 ? n : 1));
   const size_t vec_c = al(sizeof(X) * static_cast<size_t>(ncols > 0 ? ncols : 1));
-  const int64_t nb = P.nblocks > 0 ? P.nblocks : 1;
-  const size_t red_s = al(sizeof(Acc) * static_cast<size_t>(nb + red_groups(nb) + 1));
-  const size_t cnt_s = al(sizeof(unsigned) * static_cast<size_t>(1 + red_groups(nb)));
-  const size_t red_v = al(sizeof(Acc) * static_cast<size_t>(vgrid + red_groups(vgrid) + 1));
-  const size_t cnt_v = al(sizeof(unsigned) * static_cast<size_t>(1 + red_groups(vgrid)));
+  const int64_t nb = nblocks > 0 ? nblocks : 1;
+  const size_t red_s = al(sizeof(Acc) * static_cast<size_t>(nb));
+  const size_t red_v = al(sizeof(Acc) * static_cast<size_t>(vgrid));
   const size_t st_s = al(sizeof(CgState<Acc>));
-  const size_t total = 2 * vec_n + 3 * vec_c + red_s + cnt_s + 2 * (red_v + cnt_v) + al(4) + st_s;
+  const size_t total = 2 * vec_n + 3 * vec_c + red_s + 2 * red_v + st_s;
   w.mem.release();
-  char* base = static_cast<char*>(w.mem.ensure(total)); // zeroed: counters start at 0
+  char* base = static_cast<char*>(w.mem.ensure(total));
   size_t off = 0;
   auto take = [&](size_t b) {
     void* p = base + off;
@@ -49,78 +47,78 @@ void ensure_ws(CgWorkspace& w, const Pattern& P, int64_t ncols)
   w.aux = take(vec_c);
   w.aux2 = take(vec_c);
   w.red_spmv = take(red_s);
-  w.cnt_spmv = static_cast<unsigned*>(take(cnt_s));
   w.red_vec0 = take(red_v);
-  w.cnt_vec0 = static_cast<unsigned*>(take(cnt_v));
   w.red_vec1 = take(red_v);
-  w.cnt_vec1 = static_cast<unsigned*>(take(cnt_v));
-  w.cnt_xp = static_cast<unsigned*>(take(al(4)));
   w.state = take(st_s);
   w.n = n;
   w.ncols = ncols;
-  w.nblocks = P.nblocks;
+  w.nblocks = nblocks;
   w.elem = sizeof(X);
   w.vgrid = vgrid;
+}
+
+template <typename Acc>
+__global__ void cg_state_init(CgState<Acc>* st, int32_t kmax, Acc rtol2)
+{
+  CgState<Acc> s{};
+  s.kmax = kmax;
+  s.rtol2 = rtol2;
+  s.stopped = 1;
+  *st = s;
 }
 
 // Host driver of one CG solve.  x has ncols entries; b has nrows.
 template <typename V, typename X, typename Acc>
 int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t vec_dtype,
-           const X* b, X* x, int32_t kmax, Acc rtol, Acc* hist_dev, hipStream_t s)
+           const X* b, X* x, int32_t kmax, Acc rtol, Acc* hist_dev, hipStream_t s,
+           X carry = X(0))
 {
-  const Pattern& P = *A->pat;
+  Pattern& P = *A->pat;
+  const RowBlocks& rb = P.blocks_for(variant_block_nnz(A->variant));
+  ensure_ws<X, Acc>(w, P, P.ncols, rb.nblocks);
   const int64_t n = P.nrows, ncols = P.ncols;
   auto* st = static_cast<CgState<Acc>*>(w.state);
-  CgState<Acc> init{};
-  init.rtol2 = rtol * rtol;
-  init.kmax = kmax;
-  SPMVX_HIP(hipMemcpyAsync(st, &init, sizeof(init), hipMemcpyHostToDevice, s));
   X* r = static_cast<X*>(w.r);
   X* y = static_cast<X*>(w.y);
   X* p = static_cast<X*>(w.p);
-  RedWs<Acc> rs{static_cast<Acc*>(w.red_spmv), w.cnt_spmv};
-  RedWs<Acc> r0{static_cast<Acc*>(w.red_vec0), w.cnt_vec0};
-  RedWs<Acc> r1{static_cast<Acc*>(w.red_vec1), w.cnt_vec1};
+  Acc* part_s = static_cast<Acc*>(w.red_spmv);
+  Acc* part_0 = static_cast<Acc*>(w.red_vec0);
+  Acc* part_1 = static_cast<Acc*>(w.red_vec1);
   const int vg = vec_grid(n);
-  cg_init<X, Acc><<<vec_grid(ncols > n ? ncols : n), kVecBS, 0, s>>>(n, ncols, b, x, r, p, r0, st);
+  const int g0 = vec_grid(ncols > n ? ncols : n);
+  cg_state_init<Acc><<<1, 1, 0, s>>>(st, kmax, rtol * rtol);
+  cg_init<X, Acc><<<g0, kVecBS, 0, s>>>(n, ncols, b, x, r, p, part_0, carry);
+  cg_init_finalize<Acc><<<1, kFinBS, 0, s>>>(part_0, g0, st, hist_dev);
   check_launch("cg_init");
-  if (hist_dev)
-  {
-    // hist[0] = rnorm0 (read back with the state at the end)
-  }
   auto* flag = static_cast<CgState<Acc>*>(A->host_flag.ensure(sizeof(CgState<Acc>)));
-  int32_t launched = 0;
+  int32_t it = 0;
   int32_t chunk = 4;
-  while (launched < kmax)
+  while (it < kmax)
   {
-    const int32_t nb = (kmax - launched) < chunk ? (kmax - launched) : chunk;
-    for (int32_t j = 0; j < nb; ++j)
+    const int32_t last = (kmax - it) < chunk ? kmax : it + chunk;
+    for (++it; it <= last; ++it)
     {
       if (col_map)
         l2gmap_update_async(col_map, vec_dtype, p, s); // cg.cpp:55
-      if (P.nblocks > 0)
-      {
-        csr_stream<V, X, Acc, kDot><<<dim3(static_cast<unsigned>(P.nblocks)), dim3(kBS), 0, s>>>(
-            P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(), static_cast<const V*>(A->values.ptr),
-            p, y, P.blocks.as<int32_t>(), P.nblocks, nullptr, rs, st);
-        check_launch("csr_stream<kDot>");
-      }
-      cg_update_r<X, Acc><<<vg, kVecBS, 0, s>>>(n, r, y, r1, st, hist_dev);
-      cg_update_xp<X, Acc><<<vg, kVecBS, 0, s>>>(n, x, p, r, w.cnt_xp, st);
+      const int64_t nparts = launch_csr_stream<V, X, Acc, kDot>(
+          A->variant, rb.nblocks, s, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
+          static_cast<const V*>(A->values.ptr), p, y, rb.desc.as<int2>(), nullptr, part_s, st, it);
+      cg_alpha_finalize<Acc><<<1, kFinBS, 0, s>>>(part_s, nparts, st, it);
+      cg_update_r<X, Acc><<<vg, kVecBS, 0, s>>>(n, r, y, part_1, st, it);
+      cg_beta_finalize<Acc><<<1, kFinBS, 0, s>>>(part_1, vg, st, hist_dev, it);
+      cg_update_xp<X, Acc><<<vg, kVecBS, 0, s>>>(n, x, p, r, st, it);
     }
+    --it;
     check_launch("cg iteration");
-    launched += nb;
     SPMVX_HIP(hipMemcpyAsync(flag, st, sizeof(CgState<Acc>), hipMemcpyDeviceToHost, s));
     SPMVX_HIP(hipStreamSynchronize(s));
-    if (!flag->active)
+    if (flag->stopped)
       break;
     if (chunk < 32)
       chunk *= 2;
   }
   SPMVX_HIP(hipMemcpyAsync(flag, st, sizeof(CgState<Acc>), hipMemcpyDeviceToHost, s));
   SPMVX_HIP(hipStreamSynchronize(s));
-  if (hist_dev)
-    SPMVX_HIP(hipMemcpy(hist_dev, &flag->rnorm0, sizeof(Acc), hipMemcpyHostToDevice));
   return flag->k;
 }
 
@@ -166,7 +164,7 @@ int spmvx_cg(spmvx_matrix A, spmvx_l2gmap col_map, const double* b, double* x, i
   return guarded([&] {
     SPMVX_REQUIRE(A != nullptr, "NULL matrix");
     SPMVX_REQUIRE(A->dtype == SPMVX_F64, "spmv::cg needs an FP64 matrix");
-    const Pattern& P = *A->pat;
+    Pattern& P = *A->pat;
     if (col_map)
     {
       SPMVX_REQUIRE(col_map->local_size == P.nrows,
@@ -179,7 +177,6 @@ int spmvx_cg(spmvx_matrix A, spmvx_l2gmap col_map, const double* b, double* x, i
     SPMVX_REQUIRE(max_its >= 0, "max_its must be >= 0");
     SPMVX_REQUIRE((b && x) || P.nrows == 0, "NULL vector");
     hipStream_t s = as_stream(stream);
-    ensure_ws<double, double>(A->ws64, P, P.ncols);
     DevBuf hist;
     double* hd = nullptr;
     if (hist_host)
@@ -207,7 +204,6 @@ int spmvx_cg_f32(spmvx_matrix A, const float* b, float* x, int32_t max_its, doub
     SPMVX_REQUIRE(P.ncols == P.nrows, "cg_f32 needs a square matrix without ghost columns");
     SPMVX_REQUIRE(max_its >= 0, "max_its must be >= 0");
     hipStream_t s = as_stream(stream);
-    ensure_ws<float, float>(A->ws32, P, P.ncols);
     int k;
     if (A->dtype == SPMVX_F32)
       k = run_cg<float, float, float>(A, A->ws32, nullptr, SPMVX_F32, b, x, max_its,
@@ -222,7 +218,7 @@ int spmvx_cg_f32(spmvx_matrix A, const float* b, float* x, int32_t max_its, doub
 
 int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
                 int32_t max_outer, double rtol, int32_t inner_max_its, double inner_rtol,
-                spmvx_ir_stats* stats, spmvx_stream stream)
+                int32_t flags, spmvx_ir_stats* stats, spmvx_stream stream)
 {
   return guarded([&] {
     SPMVX_REQUIRE(A64 && Alow, "NULL matrix");
@@ -230,37 +226,35 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
     SPMVX_REQUIRE(Alow->dtype == SPMVX_F32 || Alow->dtype == SPMVX_BF16,
                   "inner matrix must be F32 or BF16");
     SPMVX_REQUIRE(Alow->pat == A64->pat, "inner matrix must share the pattern (create_like)");
-    const Pattern& P = *A64->pat;
+    Pattern& P = *A64->pat;
     SPMVX_REQUIRE(P.nrows == P.ncols, "refinement needs a square matrix");
     SPMVX_REQUIRE(max_outer >= 0 && max_outer <= 64, "max_outer must be in [0, 64]");
     hipStream_t s = as_stream(stream);
     const int64_t n = P.nrows;
-    ensure_ws<double, double>(A64->ws64, P, P.ncols);
-    ensure_ws<float, float>(Alow->ws32, P, P.ncols);
+    const RowBlocks& rb64 = P.blocks_for(variant_block_nnz(A64->variant));
+    ensure_ws<double, double>(A64->ws64, P, P.ncols, rb64.nblocks);
+    const RowBlocks& rb32 = P.blocks_for(variant_block_nnz(Alow->variant));
+    ensure_ws<float, float>(Alow->ws32, P, P.ncols, rb32.nblocks);
     CgWorkspace& w = A64->ws64;
     auto* st = static_cast<CgState<double>*>(w.state);
-    CgState<double> init{};
-    init.active = 1;
-    SPMVX_HIP(hipMemcpyAsync(st, &init, sizeof(init), hipMemcpyHostToDevice, s));
     double* r = static_cast<double*>(w.r);
     float* r32 = static_cast<float*>(Alow->ws32.aux);
     float* d32 = static_cast<float*>(Alow->ws32.aux2);
-    RedWs<double> rs{static_cast<double*>(w.red_spmv), w.cnt_spmv};
     const int vg = vec_grid(n);
     zero_f64<<<vg, kVecBS, 0, s>>>(x, n);
     spmvx_ir_stats local{};
     auto* flag = static_cast<CgState<double>*>(A64->host_flag.ensure(sizeof(CgState<double>)));
-    double bnorm = 0.0;
+    double bnorm = 0.0, rn_prev = 0.0;
+    const bool carry_dir = (flags & SPMVX_IR_CARRY_DIRECTION) != 0;
     int32_t outer = 0;
     for (;;)
     {
       // r = b - A x and ||r||^2 in FP64
-      if (P.nblocks > 0)
-        csr_stream<double, double, double, kResid>
-            <<<dim3(static_cast<unsigned>(P.nblocks)), dim3(kBS), 0, s>>>(
-                P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
-                static_cast<const double*>(A64->values.ptr), x, r, P.blocks.as<int32_t>(),
-                P.nblocks, b, rs, st);
+      const int64_t nparts = launch_csr_stream<double, double, double, kResid>(
+          A64->variant, rb64.nblocks, s, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
+          static_cast<const double*>(A64->values.ptr), x, r, rb64.desc.as<int2>(), b,
+          static_cast<double*>(w.red_spmv), st, 0);
+      sum_finalize<double><<<1, kFinBS, 0, s>>>(static_cast<double*>(w.red_spmv), nparts, st);
       check_launch("csr_stream<kResid>");
       SPMVX_HIP(hipMemcpyAsync(flag, st, sizeof(CgState<double>), hipMemcpyDeviceToHost, s));
       SPMVX_HIP(hipStreamSynchronize(s));
@@ -276,13 +270,19 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
       // inner FP32 solve on the normalised residual
       scale_to_f32<<<vg, kVecBS, 0, s>>>(r, r32, n, st);
       check_launch("scale_to_f32");
+      // carry: keep the inner search direction, rescaled from the previous
+      // normalisation (1/rn_prev) to this one (1/rn)
+      const float carry = (carry_dir && outer > 0) ? static_cast<float>(rn_prev / rn) : 0.0f;
+      rn_prev = rn;
       int k;
       if (Alow->dtype == SPMVX_F32)
         k = run_cg<float, float, float>(Alow, Alow->ws32, nullptr, SPMVX_F32, r32, d32,
-                                        inner_max_its, static_cast<float>(inner_rtol), nullptr, s);
+                                        inner_max_its, static_cast<float>(inner_rtol), nullptr, s,
+                                        carry);
       else
         k = run_cg<bf16_t, float, float>(Alow, Alow->ws32, nullptr, SPMVX_F32, r32, d32,
-                                         inner_max_its, static_cast<float>(inner_rtol), nullptr, s);
+                                         inner_max_its, static_cast<float>(inner_rtol), nullptr, s,
+                                         carry);
       local.inner_hist[outer] = k;
       local.inner_its += k;
       add_correction<<<vg, kVecBS, 0, s>>>(x, d32, n, st);

@@ -119,25 +119,4 @@ __device__ __forceinline__ T block_sum(T v, T* scratch)
   return s;
 }
 
-// Last-block-done detection: returns true in every thread of the block that
-// arrives last (the counter is reset for the next launch).  Partial results
-// stored before the call are visible to the last block.
-__device__ __forceinline__ bool last_block_arrive(unsigned* counter, unsigned nblocks)
-{
-  __shared__ bool am_last;
-  __syncthreads();
-  if (threadIdx.x == 0)
-  {
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    am_last = (prev == nblocks - 1);
-    if (am_last)
-      atomicExch(counter, 0u);
-  }
-  __syncthreads();
-  if (am_last)
-    __threadfence();
-  return am_last;
-}
-
 } // namespace spmvx

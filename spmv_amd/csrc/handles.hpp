@@ -1,6 +1,7 @@
 // Internal definitions of the opaque handles of the C ABI.
 #pragma once
 
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -71,24 +72,35 @@ struct PinnedBuf
   }
 };
 
+// Row-binning of a pattern for one staging chunk size: block descriptors
+// (first row, first entry), nblocks + 1 of them.
+struct RowBlocks
+{
+  DevBuf desc;
+  int64_t nblocks = 0;
+};
+
 // CSR pattern + row-binning analysis on the device; shared between a matrix
-// and its reduced-precision copies.
+// and its reduced-precision copies.  The host row pointer is kept so that
+// blocks for another chunk size can be built on demand.
 struct Pattern
 {
   int64_t nrows = 0, ncols = 0, nnz = 0;
-  DevBuf rowptr, colidx, blocks;
-  int64_t nblocks = 0;
+  DevBuf rowptr, colidx;
+  std::vector<int32_t> h_rowptr;
+  std::map<int, std::unique_ptr<RowBlocks>> blocks; // by chunk
   int device = 0;
+  const RowBlocks& blocks_for(int chunk);
 };
 
 // Row blocks for the ordered stream kernel: consecutive rows, at most
 // max_rows rows and max_nnz entries per block; a row longer than max_nnz
-// is a block of its own.
-std::vector<int32_t> row_blocks(const int32_t* rowptr, int64_t nrows, int64_t max_nnz,
-                                int64_t max_rows);
+// is a block of its own.  Returns (first row, first entry) pairs.
+std::vector<int2> row_blocks(const int32_t* rowptr, int64_t nrows, int64_t max_nnz,
+                             int64_t max_rows);
 
-// Number of groups of 256 used by grid_reduce for nb blocks.
-inline int64_t red_groups(int64_t nb) { return (nb + 255) / 256; }
+// Default kernel variant of a value type (overridable with SPMVX_SPMV_VARIANT).
+int default_variant(int32_t dtype);
 
 // Grid of the vector kernels for n elements.
 inline int vec_grid(int64_t n)
@@ -115,12 +127,8 @@ struct CgWorkspace
   void* aux2 = nullptr;
   void* state = nullptr;
   void* red_spmv = nullptr;
-  unsigned* cnt_spmv = nullptr;
   void* red_vec0 = nullptr;
-  unsigned* cnt_vec0 = nullptr;
   void* red_vec1 = nullptr;
-  unsigned* cnt_vec1 = nullptr;
-  unsigned* cnt_xp = nullptr;
   int vgrid = 0;
 };
 
@@ -129,6 +137,7 @@ struct CgWorkspace
 struct spmvx_matrix_s
 {
   int32_t dtype = SPMVX_F64;
+  int32_t variant = 0; // SpMV kernel variant (kVariants)
   std::shared_ptr<spmvx::Pattern> pat;
   spmvx::DevBuf values;
   // transposed copy for spmv_t (built on first use)

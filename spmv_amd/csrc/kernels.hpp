@@ -3,7 +3,7 @@
 // CSR SpMV, "ordered stream" kernel
 // ---------------------------------
 // The analysis (matrix.hip) cuts the rows into blocks of at most kBS rows
-// holding at most kChunk stored entries (or one longer row).  One workgroup
+// holding at most CHUNK stored entries (or one longer row).  One workgroup
 // per block:
 //   phase 1  all kBS threads walk the block's contiguous entry range with
 //            coalesced loads of colidx/values, gather x[colidx], and store the
@@ -23,8 +23,7 @@
 namespace spmvx
 {
 
-constexpr int kBS = 256;     // threads per SpMV workgroup = max rows per block
-constexpr int kChunk = 2048; // products staged in LDS per block
+constexpr int kBS = 256; // threads per SpMV workgroup = max rows per block
 
 // ---------------------------------------------------------------- values
 struct bf16_t
@@ -32,17 +31,15 @@ struct bf16_t
   uint16_t bits;
 };
 
-__device__ __forceinline__ double load_val(const double* v, int64_t k) { return v[k]; }
-__device__ __forceinline__ float load_val(const float* v, int64_t k) { return v[k]; }
-__device__ __forceinline__ float load_val(const bf16_t* v, int64_t k)
-{
-  return __uint_as_float(static_cast<uint32_t>(v[k].bits) << 16);
-}
 
 // ---------------------------------------------------------------- CG state
-// Scalars of one CG solve, resident on the device.  `active` is read at the
-// start of every kernel and only changed by the last block of the final
-// kernel of an iteration, so all blocks of a kernel agree on it.
+// Scalars of one CG solve, resident on the device.  The host enqueues the
+// kernels of iteration `it` (1-based) with `it` as an argument; they run only
+// while the solve has completed it-1 iterations and has not stopped, so the
+// iteration count is exact however many iterations are enqueued.  No kernel
+// synchronises with another inside a launch: reductions write per-block
+// partial sums and a one-workgroup finalize kernel (after the kernel
+// boundary) combines them in a fixed order.
 template <typename Acc>
 struct CgState
 {
@@ -52,97 +49,184 @@ struct CgState
   Acc beta;
   Acc rtol2;
   Acc last;    // last reduction result (residual kernels)
-  int32_t k;
+  int32_t k;       // iterations completed
   int32_t kmax;
-  int32_t active;
+  int32_t stopped; // converged or k == kmax
   int32_t converged;
 };
 
-
-// Deterministic two-level grid reduction.  Every block calls it once with its
-// block-level sum `v` (valid in thread 0) and its logical index `b` of `nb`.
-// Level 1: the last block to arrive in each group of 256 blocks sums that
-// group's partials; level 2: the last group finisher sums the group results.
-// Returns true (in all threads of that one block) with the total in *total.
-// partials: nb + ceil(nb/256) entries; counters: 1 + ceil(nb/256) entries,
-// zero on first use (each counter is reset by its last arriver).
+// Kernels of iteration `it` before the residual update run iff this holds.
 template <typename Acc>
-__device__ __forceinline__ bool grid_reduce(Acc v, int64_t b, int64_t nb, Acc* partials,
-                                            unsigned* counters, Acc* red, Acc* total)
+__device__ __forceinline__ bool cg_live(const CgState<Acc>* st, int32_t it)
 {
-  constexpr int64_t G = 256;
-  const int tid = threadIdx.x;
-  const int64_t ng = (nb + G - 1) / G;
-  const int64_t g = b / G;
-  if (tid == 0)
-    partials[b] = v;
-  const int64_t gsize = (nb - g * G) < G ? (nb - g * G) : G;
-  if (!last_block_arrive(&counters[1 + g], static_cast<unsigned>(gsize)))
-    return false;
-  Acc s = Acc(0);
-  for (int64_t i = tid; i < gsize; i += blockDim.x)
-    s += partials[g * G + i];
-  s = block_sum(s, red);
-  if (tid == 0)
-    partials[nb + g] = s;
-  if (!last_block_arrive(&counters[0], static_cast<unsigned>(ng)))
-    return false;
-  Acc t = Acc(0);
-  for (int64_t i = tid; i < ng; i += blockDim.x)
-    t += partials[nb + i];
-  t = block_sum(t, red);
-  *total = t;
-  return true;
+  return it == 0 || (!st->stopped && st->k == it - 1);
 }
 
 enum SpmvMode
 {
   kPlain = 0, // y = A x
-  kDot = 1,   // y = A p, alpha = rnorm_old / (p . y)     (CG, cg.cpp:56-62)
-  kResid = 2  // y = b - A x, last = ||y||^2              (refinement)
+  kDot = 1,   // y = A p, partial p.y                      (CG, cg.cpp:56-61)
+  kResid = 2  // y = b - A x, partial ||y||^2               (refinement)
 };
 
-// Reduction workspace of one kernel: partials[nb + nb/256 + 1], counters[...].
+// Sum of n per-block partials by one workgroup (blockDim.x = kFinBS), fixed
+// order.  Result valid in all threads.
+constexpr int kFinBS = 1024;
 template <typename Acc>
-struct RedWs
+__device__ __forceinline__ Acc finalize_sum(const Acc* __restrict__ partials, int64_t n,
+                                            Acc* red)
 {
-  Acc* partials;
-  unsigned* counters;
-};
+  Acc s0 = Acc(0), s1 = Acc(0), s2 = Acc(0), s3 = Acc(0);
+  int64_t i = threadIdx.x;
+  for (; i + 3 * kFinBS < n; i += 4 * kFinBS)
+  {
+    s0 += partials[i];
+    s1 += partials[i + kFinBS];
+    s2 += partials[i + 2 * kFinBS];
+    s3 += partials[i + 3 * kFinBS];
+  }
+  for (; i < n; i += kFinBS)
+    s0 += partials[i];
+  return block_sum((s0 + s1) + (s2 + s3), red);
+}
 
-template <typename V, typename X, typename Acc, int MODE>
+// Streaming loads of the matrix arrays (read once per product): non-temporal
+// loads keep them from displacing x, which is re-read across neighbouring
+// row blocks, from L2.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T* p)
+{
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+template <bool NT>
+__device__ __forceinline__ double ld_val(const double* v, int64_t k) { return ld_stream<NT>(v + k); }
+template <bool NT>
+__device__ __forceinline__ float ld_val(const float* v, int64_t k) { return ld_stream<NT>(v + k); }
+template <bool NT>
+__device__ __forceinline__ float ld_val(const bf16_t* v, int64_t k)
+{
+  const uint16_t b = ld_stream<NT>(reinterpret_cast<const uint16_t*>(v) + k);
+  return __uint_as_float(static_cast<uint32_t>(b) << 16);
+}
+
+// Loads N consecutive elements (N * sizeof(T) bytes, a multiple of 4 and
+// aligned to min(16, N * sizeof(T))) as 16/8/4-byte vector loads.
+template <bool NT, typename T, int N>
+__device__ __forceinline__ void load_vec(const T* p, T (&out)[N])
+{
+  constexpr int B = N * static_cast<int>(sizeof(T));
+  static_assert(B % 4 == 0 || B == 2, "vector load of whole dwords");
+  if constexpr (B >= 16)
+  {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    const u4* q = reinterpret_cast<const u4*>(p);
+#pragma unroll
+    for (int i = 0; i < B / 16; ++i)
+    {
+      const u4 v = ld_stream<NT>(q + i);
+      __builtin_memcpy(reinterpret_cast<char*>(out) + 16 * i, &v, 16);
+    }
+  }
+  else if constexpr (B == 8)
+  {
+    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+    const u2 v = ld_stream<NT>(reinterpret_cast<const u2*>(p));
+    __builtin_memcpy(out, &v, 8);
+  }
+  else if constexpr (B == 4)
+  {
+    const uint32_t v = ld_stream<NT>(reinterpret_cast<const uint32_t*>(p));
+    __builtin_memcpy(out, &v, 4);
+  }
+  else
+  {
+    const uint16_t v = ld_stream<NT>(reinterpret_cast<const uint16_t*>(p));
+    __builtin_memcpy(out, &v, 2);
+  }
+}
+
+template <typename V>
+__device__ __forceinline__ auto widen(V v)
+{
+  if constexpr (sizeof(V) == 2)
+    return __uint_as_float(static_cast<uint32_t>(reinterpret_cast<const uint16_t&>(v)) << 16);
+  else
+    return v;
+}
+
+// Kernel configuration: products staged per block (CHUNK), consecutive
+// entries each thread loads with one vector load per array (VEC; the entry
+// range of a block is widened down to a multiple of VEC, the arrays are
+// padded, and entries outside the block are skipped), independent vector
+// steps in flight per thread (UNROLL), non-temporal matrix loads (NT).  Row
+// blocks hold at most kBS rows and CHUNK entries.  Block descriptors are
+// (first row, first entry) pairs.
+template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, int UNROLL, bool NT>
 __global__ __launch_bounds__(kBS) void csr_stream(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
-    const int32_t* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
-    RedWs<Acc> ws, CgState<Acc>* st)
+    const int2* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
+    Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it)
 {
-  if (MODE != kPlain && !st->active)
+  if (MODE != kPlain && !cg_live(st, it))
     return;
-  __shared__ Acc prod[kChunk];
+  __shared__ Acc prod[CHUNK];
   __shared__ Acc red[16];
 
   const int64_t bi = xcd_block(blockIdx.x, nblocks);
-  const int32_t r0 = blocks[bi], r1 = blocks[bi + 1];
-  const int32_t k0 = rowptr[r0], k1 = rowptr[r1];
+  const int2 d0 = blocks[bi], d1 = blocks[bi + 1];
+  const int32_t r0 = d0.x, r1 = d1.x;
+  const int32_t k0 = d0.y, k1 = d1.y;
   const int tid = threadIdx.x;
+  const int32_t row = r0 + tid;
+  // row extents, loaded early (independent of the staging below)
+  int32_t ra = 0, re = 0;
+  if (row < r1)
+  {
+    ra = rowptr[row];
+    re = rowptr[row + 1];
+  }
   Acc part = Acc(0);
 
-  if (k1 - k0 <= kChunk)
+  if (k1 - k0 <= CHUNK)
   {
-    const int n = k1 - k0;
-    for (int t = tid; t < n; t += kBS)
+    const int32_t kb = k0 & ~(VEC - 1);
+    constexpr int STEP = kBS * VEC;
+    for (int32_t g0 = kb; g0 < k1; g0 += STEP * UNROLL)
     {
-      const int64_t k = k0 + t;
-      prod[t] = static_cast<Acc>(load_val(val, k)) * static_cast<Acc>(x[colidx[k]]);
+      int32_t c[UNROLL][VEC];
+      V v[UNROLL][VEC];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u)
+      {
+        const int32_t g = g0 + u * STEP + tid * VEC;
+        if (g < k1)
+        {
+          load_vec<NT>(colidx + g, c[u]);
+          load_vec<NT>(val + g, v[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u)
+      {
+        const int32_t g = g0 + u * STEP + tid * VEC;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j)
+        {
+          const int32_t k = g + j;
+          if (k >= k0 && k < k1)
+            prod[k - k0] = static_cast<Acc>(widen(v[u][j])) * static_cast<Acc>(x[c[u][j]]);
+        }
+      }
     }
     __syncthreads();
-    const int32_t row = r0 + tid;
     if (row < r1)
     {
-      const int a = rowptr[row] - k0, e = rowptr[row + 1] - k0;
       Acc s = Acc(0);
-      for (int q = a; q < e; ++q)
+      for (int q = ra - k0; q < re - k0; ++q)
         s += prod[q];
       if (MODE == kResid)
       {
@@ -161,17 +245,17 @@ __global__ __launch_bounds__(kBS) void csr_stream(
   }
   else
   {
-    // One row longer than kChunk: staged chunk by chunk, summed in order by
+    // One row longer than CHUNK: staged chunk by chunk, summed in order by
     // thread 0.
     Acc s = Acc(0);
-    for (int64_t c0 = k0; c0 < k1; c0 += kChunk)
+    for (int64_t c0 = k0; c0 < k1; c0 += CHUNK)
     {
-      const int n = static_cast<int>((k1 - c0) < kChunk ? (k1 - c0) : kChunk);
+      const int n = static_cast<int>((k1 - c0) < CHUNK ? (k1 - c0) : CHUNK);
       __syncthreads();
       for (int t = tid; t < n; t += kBS)
       {
         const int64_t k = c0 + t;
-        prod[t] = static_cast<Acc>(load_val(val, k)) * static_cast<Acc>(x[colidx[k]]);
+        prod[t] = static_cast<Acc>(ld_val<NT>(val, k)) * static_cast<Acc>(x[colidx[k]]);
       }
       __syncthreads();
       if (tid == 0)
@@ -199,25 +283,309 @@ __global__ __launch_bounds__(kBS) void csr_stream(
   if (MODE == kPlain)
     return;
   part = block_sum(part, red);
-  Acc total;
-  if (grid_reduce(part, bi, nblocks, ws.partials, ws.counters, red, &total) && tid == 0)
+  if (tid == 0)
+    partials[bi] = part;
+}
+
+// Software-pipelined variant.  A fixed grid of workgroups (about as many as
+// fit on the chip at once) each walks a contiguous range of row blocks, the
+// ranges of one XCD being adjacent.  While the products of block i are
+// formed (x gathers), summed and stored, the column indices, values and row
+// extents of block i+1 are already in flight in registers, so the matrix
+// stream never pauses.  Blocks hold at most CHUNK - 8 entries here, so that
+// a block's range widened down to a multiple of VEC is covered by PER vector
+// steps.  Per-row arithmetic is the same as csr_stream's (bit-identical).
+template <typename V, int VEC, int PER>
+struct PipeStage
+{
+  int32_t c[PER][VEC];
+  V v[PER][VEC];
+  int32_t ra, re;
+};
+
+template <bool NT, typename V, int VEC, int PER>
+__device__ __forceinline__ void pipe_issue(PipeStage<V, VEC, PER>& S,
+                                           const int32_t* __restrict__ rowptr,
+                                           const int32_t* __restrict__ colidx,
+                                           const V* __restrict__ val, int2 d0, int2 d1,
+                                           int long_limit)
+{
+  const int tid = threadIdx.x;
+  const int32_t row = d0.x + tid;
+  if (row < d1.x)
   {
-    if (MODE == kDot)
-      st->alpha = st->rnorm_old / total;
-    else
-      st->last = total;
+    S.ra = rowptr[row];
+    S.re = rowptr[row + 1];
   }
+  if (d1.y - d0.y > long_limit)
+    return; // long row: loaded directly when processed
+  const int32_t kb = d0.y & ~(VEC - 1);
+#pragma unroll
+  for (int u = 0; u < PER; ++u)
+  {
+    const int32_t g = kb + (u * kBS + tid) * VEC;
+    if (g < d1.y)
+    {
+      load_vec<NT>(colidx + g, S.c[u]);
+      load_vec<NT>(val + g, S.v[u]);
+    }
+  }
+}
+
+// x gathers of a staged block.  Issued before the next block's loads so that
+// waiting for them (vmcnt counts loads in issue order) does not also wait for
+// the prefetch.
+template <typename V, typename X, int VEC, int PER>
+__device__ __forceinline__ void pipe_gather(const PipeStage<V, VEC, PER>& S,
+                                            const X* __restrict__ x, int2 d0, int2 d1,
+                                            X (&xg)[PER][VEC], int long_limit)
+{
+  const int32_t k0 = d0.y, k1 = d1.y;
+  if (k1 - k0 > long_limit)
+    return;
+  const int32_t kb = k0 & ~(VEC - 1);
+#pragma unroll
+  for (int u = 0; u < PER; ++u)
+  {
+    const int32_t g = kb + (u * kBS + static_cast<int32_t>(threadIdx.x)) * VEC;
+#pragma unroll
+    for (int j = 0; j < VEC; ++j)
+    {
+      const int32_t k = g + j;
+      xg[u][j] = (k >= k0 && k < k1) ? x[S.c[u][j]] : X(0);
+    }
+  }
+}
+
+template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, int PER, bool NT>
+__device__ __forceinline__ void pipe_compute(const PipeStage<V, VEC, PER>& S,
+                                             const X (&xg)[PER][VEC],
+                                             const int32_t* __restrict__ colidx,
+                                             const V* __restrict__ val,
+                                             const X* __restrict__ x, X* __restrict__ y,
+                                             const X* __restrict__ b, int2 d0, int2 d1,
+                                             Acc* prod, Acc& part, int long_limit)
+{
+  const int tid = threadIdx.x;
+  const int32_t r0 = d0.x, r1 = d1.x, k0 = d0.y, k1 = d1.y;
+  const int32_t row = r0 + tid;
+  if (k1 - k0 <= long_limit)
+  {
+    const int32_t kb = k0 & ~(VEC - 1);
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+    {
+      const int32_t g = kb + (u * kBS + tid) * VEC;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j)
+      {
+        const int32_t k = g + j;
+        if (k >= k0 && k < k1)
+          prod[k - k0] = static_cast<Acc>(widen(S.v[u][j])) * static_cast<Acc>(xg[u][j]);
+      }
+    }
+    __syncthreads();
+    if (row < r1)
+    {
+      Acc s = Acc(0);
+      for (int q = S.ra - k0; q < S.re - k0; ++q)
+        s += prod[q];
+      if (MODE == kResid)
+      {
+        const X yi = b[row] - static_cast<X>(s);
+        y[row] = yi;
+        part += static_cast<Acc>(yi) * static_cast<Acc>(yi);
+      }
+      else
+      {
+        const X yi = static_cast<X>(s);
+        y[row] = yi;
+        if (MODE == kDot)
+          part += static_cast<Acc>(x[row]) * static_cast<Acc>(yi);
+      }
+    }
+    __syncthreads();
+  }
+  else
+  {
+    // one long row, staged chunk by chunk and summed in order by thread 0
+    Acc s = Acc(0);
+    for (int64_t c0 = k0; c0 < k1; c0 += CHUNK)
+    {
+      const int n = static_cast<int>((k1 - c0) < CHUNK ? (k1 - c0) : CHUNK);
+      for (int t = tid; t < n; t += kBS)
+      {
+        const int64_t k = c0 + t;
+        prod[t] = static_cast<Acc>(ld_val<NT>(val, k)) * static_cast<Acc>(x[colidx[k]]);
+      }
+      __syncthreads();
+      if (tid == 0)
+        for (int q = 0; q < n; ++q)
+          s += prod[q];
+      __syncthreads();
+    }
+    if (tid == 0)
+    {
+      if (MODE == kResid)
+      {
+        const X yi = b[r0] - static_cast<X>(s);
+        y[r0] = yi;
+        part += static_cast<Acc>(yi) * static_cast<Acc>(yi);
+      }
+      else
+      {
+        const X yi = static_cast<X>(s);
+        y[r0] = yi;
+        if (MODE == kDot)
+          part += static_cast<Acc>(x[r0]) * static_cast<Acc>(yi);
+      }
+    }
+  }
+}
+
+template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, bool NT>
+__global__ __launch_bounds__(kBS) void csr_pipe(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+    const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
+    const int2* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
+    Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it)
+{
+  if (MODE != kPlain && !cg_live(st, it))
+    return;
+  constexpr int PER = CHUNK / (kBS * VEC);
+  static_assert(PER * kBS * VEC == CHUNK, "CHUNK must be a multiple of kBS*VEC");
+  constexpr int kLong = CHUNK - 8;
+  __shared__ Acc prod[CHUNK];
+  __shared__ Acc red[16];
+  const int64_t w = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t bb = nblocks * w / gridDim.x, be = nblocks * (w + 1) / gridDim.x;
+  Acc part = Acc(0);
+  PipeStage<V, VEC, PER> A, B;
+  X xg[PER][VEC];
+  if (bb < be)
+  {
+    int2 a0 = blocks[bb], a1 = blocks[bb + 1];
+    pipe_issue<NT>(A, rowptr, colidx, val, a0, a1, kLong);
+    int64_t bi = bb;
+    for (;;)
+    {
+      int2 n0 = a1, n1 = a1;
+      pipe_gather(A, x, a0, a1, xg, kLong);
+      if (bi + 1 < be)
+      {
+        n1 = blocks[bi + 2];
+        pipe_issue<NT>(B, rowptr, colidx, val, n0, n1, kLong);
+      }
+      pipe_compute<V, X, Acc, MODE, CHUNK, VEC, PER, NT>(A, xg, colidx, val, x, y, b, a0, a1,
+                                                          prod, part, kLong);
+      if (++bi >= be)
+        break;
+      a0 = n0;
+      a1 = n1;
+      pipe_gather(B, x, a0, a1, xg, kLong);
+      if (bi + 1 < be)
+      {
+        n0 = a1;
+        n1 = blocks[bi + 2];
+        pipe_issue<NT>(A, rowptr, colidx, val, n0, n1, kLong);
+      }
+      pipe_compute<V, X, Acc, MODE, CHUNK, VEC, PER, NT>(B, xg, colidx, val, x, y, b, a0, a1,
+                                                          prod, part, kLong);
+      if (++bi >= be)
+        break;
+      a0 = n0;
+      a1 = n1;
+    }
+  }
+  if (MODE == kPlain)
+    return;
+  part = block_sum(part, red);
+  if (threadIdx.x == 0)
+    partials[w] = part;
+}
+
+// Kernel variants selectable per matrix: {kind (0 one block per workgroup,
+// 1 pipelined), chunk, vec, unroll, non-temporal}.
+struct SpmvVariant
+{
+  int kind;
+  int chunk;
+  int vec;
+  int unroll;
+  bool nt;
+};
+#define SPMVX_VARIANTS(X)                                                                    \
+  X(0, 0, 2048, 1, 1, false)                                                                 \
+  X(1, 0, 2048, 1, 4, true)                                                                  \
+  X(2, 0, 2048, 2, 2, true)                                                                  \
+  X(3, 0, 2048, 4, 1, true)                                                                  \
+  X(4, 0, 2048, 4, 2, true)                                                                  \
+  X(5, 1, 2048, 2, 1, true)                                                                  \
+  X(6, 1, 2048, 4, 1, true)                                                                  \
+  X(7, 1, 2048, 8, 1, true)                                                                  \
+  X(8, 1, 4096, 4, 1, true)                                                                  \
+  X(9, 1, 1024, 4, 1, true)                                                                  \
+  X(10, 1, 2048, 4, 1, false)
+constexpr int kNumVariants = 11;
+#define SPMVX_VARIANT_ENTRY(i, K, C, V, U, N) {K, C, V, U, N},
+constexpr SpmvVariant kVariants[kNumVariants] = {SPMVX_VARIANTS(SPMVX_VARIANT_ENTRY)};
+#undef SPMVX_VARIANT_ENTRY
+
+// Entries per row block of a variant (pipelined kernels keep 8 spare slots).
+inline int variant_block_nnz(int var)
+{
+  return kVariants[var].kind == 1 ? kVariants[var].chunk - 8 : kVariants[var].chunk;
+}
+
+// Launches the SpMV kernel of variant `var` over `nblocks` row blocks (built
+// with variant_block_nnz(var) entries per block); returns the number of
+// partial sums written (MODE != kPlain).
+int pipe_grid(const void* kernel, int64_t nblocks);
+
+template <typename V, typename X, typename Acc, int MODE>
+int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t* rowptr,
+                          const int32_t* colidx, const V* val, const X* x, X* y,
+                          const int2* blocks, const X* b, Acc* partials,
+                          const CgState<Acc>* st, int32_t it)
+{
+  if (nblocks <= 0)
+    return 0;
+  int64_t grid = nblocks;
+#define SPMVX_CASE(i, K, C, VV, U, N)                                                        \
+  case i:                                                                                    \
+    if constexpr (K == 0)                                                                    \
+      csr_stream<V, X, Acc, MODE, C, VV, U, N><<<dim3(static_cast<unsigned>(grid)),          \
+                                                 dim3(kBS), 0, s>>>(                        \
+          rowptr, colidx, val, x, y, blocks, nblocks, b, partials, st, it);                  \
+    else                                                                                     \
+    {                                                                                        \
+      grid = pipe_grid(reinterpret_cast<const void*>(&csr_pipe<V, X, Acc, MODE, C, VV, N>),  \
+                       nblocks);                                                             \
+      csr_pipe<V, X, Acc, MODE, C, VV, N><<<dim3(static_cast<unsigned>(grid)), dim3(kBS), 0, \
+                                            s>>>(rowptr, colidx, val, x, y, blocks, nblocks, \
+                                                 b, partials, st, it);                       \
+    }                                                                                        \
+    break;
+  switch (var)
+  {
+    SPMVX_VARIANTS(SPMVX_CASE)
+  default: break;
+  }
+#undef SPMVX_CASE
+  return grid;
 }
 
 // ---------------------------------------------------------------- CG vectors
 constexpr int kVecBS = 256;
 
-// x = 0, r = b, p[0:n) = b, p[n:ncols) = 0, rnorm0 = ||b||^2 (cg.cpp:31-44)
+// x = 0, r = b, p[0:n) = b, p[n:ncols) = 0, partial ||b||^2 (cg.cpp:31-44).
+// With carry > 0 the previous direction is kept, rescaled (p = carry * p):
+// the residual-replacement restart of the refinement loop.
 template <typename X, typename Acc>
 __global__ __launch_bounds__(kVecBS) void cg_init(int64_t n, int64_t ncols,
                                                   const X* __restrict__ b, X* __restrict__ x,
                                                   X* __restrict__ r, X* __restrict__ p,
-                                                  RedWs<Acc> ws, CgState<Acc>* st)
+                                                  Acc* __restrict__ partials, X carry)
 {
   __shared__ Acc red[16];
   Acc part = Acc(0);
@@ -229,32 +597,70 @@ __global__ __launch_bounds__(kVecBS) void cg_init(int64_t n, int64_t ncols,
     {
       const X bi = b[i];
       r[i] = bi;
-      p[i] = bi;
+      p[i] = carry > X(0) ? carry * p[i] : bi;
       part += static_cast<Acc>(bi) * static_cast<Acc>(bi);
     }
     else
       p[i] = X(0);
   }
   part = block_sum(part, red);
-  Acc total;
-  if (grid_reduce(part, blockIdx.x, gridDim.x, ws.partials, ws.counters, red, &total)
-      && threadIdx.x == 0)
+  if (threadIdx.x == 0)
+    partials[blockIdx.x] = part;
+}
+
+// rnorm0 = rnorm_old = sum, k = 0 (cg.cpp:42-47)
+template <typename Acc>
+__global__ __launch_bounds__(kFinBS) void cg_init_finalize(const Acc* __restrict__ partials,
+                                                           int64_t n, CgState<Acc>* st,
+                                                           Acc* __restrict__ hist)
+{
+  __shared__ Acc red[16];
+  const Acc s = finalize_sum(partials, n, red);
+  if (threadIdx.x == 0)
   {
-    st->rnorm0 = total;
-    st->rnorm_old = total;
+    st->rnorm0 = s;
+    st->rnorm_old = s;
     st->k = 0;
     st->converged = 0;
-    st->active = st->kmax > 0 ? 1 : 0;
+    st->stopped = st->kmax > 0 ? 0 : 1;
+    if (hist)
+      hist[0] = s;
   }
 }
 
-// r -= alpha y; rnorm_new = ||r||^2; beta; k++; convergence (cg.cpp:66-79)
+// alpha = rnorm_old / (p.y)  (cg.cpp:59-62)
+template <typename Acc>
+__global__ __launch_bounds__(kFinBS) void cg_alpha_finalize(const Acc* __restrict__ partials,
+                                                            int64_t n, CgState<Acc>* st,
+                                                            int32_t it)
+{
+  if (!cg_live(st, it))
+    return;
+  __shared__ Acc red[16];
+  const Acc s = finalize_sum(partials, n, red);
+  if (threadIdx.x == 0)
+    st->alpha = st->rnorm_old / s;
+}
+
+// last = sum (refinement residual norm)
+template <typename Acc>
+__global__ __launch_bounds__(kFinBS) void sum_finalize(const Acc* __restrict__ partials,
+                                                       int64_t n, CgState<Acc>* st)
+{
+  __shared__ Acc red[16];
+  const Acc s = finalize_sum(partials, n, red);
+  if (threadIdx.x == 0)
+    st->last = s;
+}
+
+// r -= alpha y; partial ||r||^2 (cg.cpp:66,69)
 template <typename X, typename Acc>
 __global__ __launch_bounds__(kVecBS) void cg_update_r(int64_t n, X* __restrict__ r,
-                                                      const X* __restrict__ y, RedWs<Acc> ws,
-                                                      CgState<Acc>* st, Acc* __restrict__ hist)
+                                                      const X* __restrict__ y,
+                                                      Acc* __restrict__ partials,
+                                                      const CgState<Acc>* st, int32_t it)
 {
-  if (!st->active)
+  if (!cg_live(st, it))
     return;
   __shared__ Acc red[16];
   const X alpha = static_cast<X>(st->alpha);
@@ -267,28 +673,41 @@ __global__ __launch_bounds__(kVecBS) void cg_update_r(int64_t n, X* __restrict__
     part += static_cast<Acc>(ri) * static_cast<Acc>(ri);
   }
   part = block_sum(part, red);
-  Acc s;
-  if (grid_reduce(part, blockIdx.x, gridDim.x, ws.partials, ws.counters, red, &s)
-      && threadIdx.x == 0)
+  if (threadIdx.x == 0)
+    partials[blockIdx.x] = part;
+}
+
+// rnorm_new, beta, k = it, convergence (cg.cpp:69-79)
+template <typename Acc>
+__global__ __launch_bounds__(kFinBS) void cg_beta_finalize(const Acc* __restrict__ partials,
+                                                           int64_t n, CgState<Acc>* st,
+                                                           Acc* __restrict__ hist, int32_t it)
+{
+  if (!cg_live(st, it))
+    return;
+  __shared__ Acc red[16];
+  const Acc s = finalize_sum(partials, n, red);
+  if (threadIdx.x == 0)
   {
     st->beta = s / st->rnorm_old;
     st->rnorm_old = s;
-    st->k += 1;
+    st->k = it;
     if (hist)
-      hist[st->k] = s;
+      hist[it] = s;
     st->converged = (s / st->rnorm0 < st->rtol2) ? 1 : 0;
+    st->stopped = (st->converged || it >= st->kmax) ? 1 : 0;
   }
 }
 
-// x += alpha p; p = beta p + r; stop after this iteration if converged or
-// k == kmax (cg.cpp:65,76-79)
+// x += alpha p; p = beta p + r (cg.cpp:65,76): runs iff iteration `it`
+// completed its residual update (also on the iteration that stops).
 template <typename X, typename Acc>
 __global__ __launch_bounds__(kVecBS) void cg_update_xp(int64_t n, X* __restrict__ x,
                                                        X* __restrict__ p,
                                                        const X* __restrict__ r,
-                                                       unsigned* counter, CgState<Acc>* st)
+                                                       const CgState<Acc>* st, int32_t it)
 {
-  if (!st->active)
+  if (st->k != it)
     return;
   const X alpha = static_cast<X>(st->alpha);
   const X beta = static_cast<X>(st->beta);
@@ -298,11 +717,6 @@ __global__ __launch_bounds__(kVecBS) void cg_update_xp(int64_t n, X* __restrict_
     const X pi = p[i];
     x[i] = x[i] + alpha * pi;
     p[i] = pi * beta + r[i];
-  }
-  if (last_block_arrive(counter, gridDim.x) && threadIdx.x == 0)
-  {
-    if (st->converged || st->k >= st->kmax)
-      st->active = 0;
   }
 }
 

@@ -1,6 +1,7 @@
 // Matrix handle: CSR upload + row-binning analysis, SpMV, transpose SpMV,
 // SDDMM, value updates (spmv/Matrix.h:30-53, spmv/Matrix.cpp:13-211).
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "handles.hpp"
@@ -8,12 +9,12 @@
 namespace spmvx
 {
 
-std::vector<int32_t> row_blocks(const int32_t* rowptr, int64_t nrows, int64_t max_nnz,
-                                int64_t max_rows)
+std::vector<int2> row_blocks(const int32_t* rowptr, int64_t nrows, int64_t max_nnz,
+                             int64_t max_rows)
 {
-  std::vector<int32_t> b;
+  std::vector<int2> b;
   b.reserve(static_cast<size_t>(nrows / 16 + 2));
-  b.push_back(0);
+  b.push_back(make_int2(0, rowptr[0]));
   int64_t r = 0;
   while (r < nrows)
   {
@@ -29,9 +30,64 @@ std::vector<int32_t> row_blocks(const int32_t* rowptr, int64_t nrows, int64_t ma
     }
     if (r == start)
       ++r; // a single row longer than max_nnz
-    b.push_back(static_cast<int32_t>(r));
+    b.push_back(make_int2(static_cast<int32_t>(r), rowptr[r]));
   }
   return b;
+}
+
+const RowBlocks& Pattern::blocks_for(int chunk)
+{
+  auto it = blocks.find(chunk);
+  if (it != blocks.end())
+    return *it->second;
+  auto rb = std::make_unique<RowBlocks>();
+  std::vector<int2> b = row_blocks(h_rowptr.data(), nrows, chunk, kBS);
+  rb->nblocks = static_cast<int64_t>(b.size()) - 1;
+  rb->desc.ensure(sizeof(int2) * b.size());
+  SPMVX_HIP(hipMemcpy(rb->desc.ptr, b.data(), sizeof(int2) * b.size(), hipMemcpyHostToDevice));
+  const RowBlocks& ref = *rb;
+  blocks[chunk] = std::move(rb);
+  return ref;
+}
+
+int pipe_grid(const void* kernel, int64_t nblocks)
+{
+  // all workgroups resident at once: CUs x the kernel's occupancy
+  static int ncu = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess
+        || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      n = 256;
+    return n > 0 ? n : 256;
+  }();
+  static std::map<const void*, int> occ;
+  auto it = occ.find(kernel);
+  if (it == occ.end())
+  {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBS, 0) != hipSuccess
+        || per_cu <= 0)
+      per_cu = 4;
+    it = occ.emplace(kernel, per_cu).first;
+  }
+  const int64_t g = static_cast<int64_t>(ncu) * it->second;
+  return static_cast<int>(g < nblocks ? g : nblocks);
+}
+
+int default_variant(int32_t dtype)
+{
+  // measured best on gfx950 for stencil matrices (bench sweep): FP64 vec 2 x
+  // unroll 2, FP32 vec 4, BF16 vec 4 x unroll 2; SPMVX_SPMV_VARIANT overrides
+  static int forced = [] {
+    const char* e = std::getenv("SPMVX_SPMV_VARIANT");
+    if (!e)
+      return -1;
+    const int x = std::atoi(e);
+    return (x >= 0 && x < kNumVariants) ? x : -1;
+  }();
+  if (forced >= 0)
+    return forced;
+  return dtype == SPMVX_F64 ? 2 : (dtype == SPMVX_F32 ? 3 : 4);
 }
 
 namespace
@@ -114,7 +170,7 @@ __global__ void copy_f64_kernel(double* __restrict__ dst, const double* __restri
 // an entry found by binary search in the block's row pointers (LDS).
 __global__ __launch_bounds__(kBS) void sddmm_kernel(const int32_t* __restrict__ rowptr,
                                                     const int32_t* __restrict__ colidx,
-                                                    const int32_t* __restrict__ blocks,
+                                                    const int2* __restrict__ blocks,
                                                     int64_t nblocks, double alpha,
                                                     const double* __restrict__ u,
                                                     const double* __restrict__ v,
@@ -122,7 +178,7 @@ __global__ __launch_bounds__(kBS) void sddmm_kernel(const int32_t* __restrict__ 
 {
   __shared__ int32_t rp[kBS + 1];
   const int64_t bi = xcd_block(blockIdx.x, nblocks);
-  const int32_t r0 = blocks[bi], r1 = blocks[bi + 1];
+  const int32_t r0 = blocks[bi].x, r1 = blocks[bi + 1].x;
   const int nr = r1 - r0;
   for (int t = threadIdx.x; t <= nr; t += kBS)
     rp[t] = rowptr[r0 + t];
@@ -144,26 +200,25 @@ __global__ __launch_bounds__(kBS) void sddmm_kernel(const int32_t* __restrict__ 
 }
 
 template <typename V, typename X, typename Acc>
-void launch_plain(const Pattern& P, const void* vals, const void* x, void* y, hipStream_t s)
+void launch_plain(int var, Pattern& P, const void* vals, const void* x, void* y, hipStream_t s)
 {
-  if (P.nblocks == 0)
-    return;
-  csr_stream<V, X, Acc, kPlain><<<dim3(static_cast<unsigned>(P.nblocks)), dim3(kBS), 0, s>>>(
-      P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(), static_cast<const V*>(vals),
-      static_cast<const X*>(x), static_cast<X*>(y), P.blocks.as<int32_t>(), P.nblocks,
-      nullptr, RedWs<Acc>{nullptr, nullptr}, nullptr);
-  check_launch("csr_stream");
+  const RowBlocks& rb = P.blocks_for(variant_block_nnz(var));
+  launch_csr_stream<V, X, Acc, kPlain>(var, rb.nblocks, s, P.rowptr.as<int32_t>(),
+                                       P.colidx.as<int32_t>(), static_cast<const V*>(vals),
+                                       static_cast<const X*>(x), static_cast<X*>(y),
+                                       rb.desc.as<int2>(), nullptr, nullptr, nullptr, 0);
+  check_launch("csr SpMV");
 }
 
-void launch_spmv(int32_t dtype, const Pattern& P, const void* vals, const void* x, void* y,
+void launch_spmv(int32_t dtype, int var, Pattern& P, const void* vals, const void* x, void* y,
                  hipStream_t s)
 {
   if (dtype == SPMVX_F64)
-    launch_plain<double, double, double>(P, vals, x, y, s);
+    launch_plain<double, double, double>(var, P, vals, x, y, s);
   else if (dtype == SPMVX_F32)
-    launch_plain<float, float, float>(P, vals, x, y, s);
+    launch_plain<float, float, float>(var, P, vals, x, y, s);
   else
-    launch_plain<bf16_t, float, float>(P, vals, x, y, s);
+    launch_plain<bf16_t, float, float>(var, P, vals, x, y, s);
 }
 
 std::shared_ptr<Pattern> upload_pattern(int64_t nrows, int64_t ncols, int64_t nnz,
@@ -176,13 +231,11 @@ std::shared_ptr<Pattern> upload_pattern(int64_t nrows, int64_t ncols, int64_t nn
   SPMVX_HIP(hipGetDevice(&P->device));
   P->rowptr.ensure(sizeof(int32_t) * static_cast<size_t>(nrows + 1));
   SPMVX_HIP(hipMemcpy(P->rowptr.ptr, rowptr, sizeof(int32_t) * (nrows + 1), hipMemcpyHostToDevice));
-  P->colidx.ensure(sizeof(int32_t) * static_cast<size_t>(nnz > 0 ? nnz : 1));
+  P->colidx.ensure(sizeof(int32_t) * static_cast<size_t>(nnz + 16));
   if (nnz > 0)
     SPMVX_HIP(hipMemcpy(P->colidx.ptr, colidx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
-  std::vector<int32_t> b = row_blocks(rowptr, nrows, kChunk, kBS);
-  P->nblocks = static_cast<int64_t>(b.size()) - 1;
-  P->blocks.ensure(sizeof(int32_t) * b.size());
-  SPMVX_HIP(hipMemcpy(P->blocks.ptr, b.data(), sizeof(int32_t) * b.size(), hipMemcpyHostToDevice));
+  P->h_rowptr.assign(rowptr, rowptr + nrows + 1);
+  (void)P->blocks_for(variant_block_nnz(default_variant(SPMVX_F64)));
   return P;
 }
 
@@ -231,7 +284,7 @@ void build_transpose(spmvx_matrix_s* A)
       std::memcpy(&tv[es * dst], &v[es * k], es);
     }
   A->tpat = upload_pattern(P.ncols, P.nrows, P.nnz, trp.data(), tci.data());
-  A->tvalues.ensure(es * static_cast<size_t>(P.nnz > 0 ? P.nnz : 1));
+  A->tvalues.ensure(es * static_cast<size_t>(P.nnz + 16));
   if (P.nnz)
     SPMVX_HIP(hipMemcpy(A->tvalues.ptr, tv.data(), es * P.nnz, hipMemcpyHostToDevice));
 }
@@ -255,8 +308,9 @@ int spmvx_matrix_create(int32_t dtype, int64_t nrows, int64_t ncols, int64_t nnz
     SPMVX_REQUIRE(nnz == 0 || values != nullptr, "values is NULL");
     auto A = std::make_unique<spmvx_matrix_s>();
     A->dtype = dtype;
+    A->variant = default_variant(dtype);
     A->pat = upload_pattern(nrows, ncols, nnz, rowptr, colidx);
-    A->values.ensure(es * static_cast<size_t>(nnz > 0 ? nnz : 1));
+    A->values.ensure(es * static_cast<size_t>(nnz + 16));
     if (nnz > 0)
     {
       if (dtype == SPMVX_F64)
@@ -287,9 +341,10 @@ int spmvx_matrix_create_like(spmvx_matrix base, int32_t dtype, spmvx_matrix* out
     const size_t es = dtype_size(dtype);
     auto A = std::make_unique<spmvx_matrix_s>();
     A->dtype = dtype;
+    A->variant = default_variant(dtype);
     A->pat = base->pat;
     const int64_t nnz = base->pat->nnz;
-    A->values.ensure(es * static_cast<size_t>(nnz > 0 ? nnz : 1));
+    A->values.ensure(es * static_cast<size_t>(nnz + 16));
     if (nnz > 0)
     {
       convert_values<<<1024, 256>>>(base->values.as<double>(), A->values.ptr, nnz, dtype);
@@ -314,7 +369,7 @@ int spmvx_matrix_info(spmvx_matrix A, int32_t* dtype, int64_t* nrows, int64_t* n
     if (nrows) *nrows = A->pat->nrows;
     if (ncols) *ncols = A->pat->ncols;
     if (nnz) *nnz = A->pat->nnz;
-    if (nblocks) *nblocks = A->pat->nblocks;
+    if (nblocks) *nblocks = A->pat->blocks_for(variant_block_nnz(A->variant)).nblocks;
   });
 }
 
@@ -345,13 +400,24 @@ int spmvx_matrix_set_values(spmvx_matrix A, const double* values_dev, spmvx_stre
   });
 }
 
+int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    SPMVX_REQUIRE(variant >= -1 && variant < kNumVariants, "unknown kernel variant");
+    A->variant = variant < 0 ? default_variant(A->dtype) : variant;
+  });
+}
+
+int spmvx_num_kernels(void) { return kNumVariants; }
+
 int spmvx_spmv(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
 {
   return guarded([&] {
     SPMVX_REQUIRE(A != nullptr, "NULL matrix");
     SPMVX_REQUIRE(A->pat->nrows == 0 || y != nullptr, "y is NULL");
     SPMVX_REQUIRE(A->pat->nnz == 0 || x != nullptr, "x is NULL");
-    launch_spmv(A->dtype, *A->pat, A->values.ptr, x, y, as_stream(stream));
+    launch_spmv(A->dtype, A->variant, *A->pat, A->values.ptr, x, y, as_stream(stream));
   });
 }
 
@@ -362,7 +428,7 @@ int spmvx_spmv_t(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
     if (!A->tpat)
       build_transpose(A);
     SPMVX_REQUIRE(A->pat->ncols == 0 || y != nullptr, "y is NULL");
-    launch_spmv(A->dtype, *A->tpat, A->tvalues.ptr, x, y, as_stream(stream));
+    launch_spmv(A->dtype, A->variant, *A->tpat, A->tvalues.ptr, x, y, as_stream(stream));
   });
 }
 
@@ -372,12 +438,13 @@ int spmvx_sddmm(spmvx_matrix A, double alpha, const double* u, const double* v, 
   return guarded([&] {
     SPMVX_REQUIRE(A != nullptr, "NULL matrix");
     SPMVX_REQUIRE(A->dtype == SPMVX_F64, "sddmm needs an FP64 matrix");
-    const Pattern& P = *A->pat;
-    if (P.nblocks == 0 || P.nnz == 0)
+    Pattern& P = *A->pat;
+    const RowBlocks& rb = P.blocks_for(variant_block_nnz(A->variant));
+    if (rb.nblocks == 0 || P.nnz == 0)
       return;
     SPMVX_REQUIRE(u && v && out, "NULL vector");
-    sddmm_kernel<<<dim3(static_cast<unsigned>(P.nblocks)), dim3(kBS), 0, as_stream(stream)>>>(
-        P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(), P.blocks.as<int32_t>(), P.nblocks, alpha,
+    sddmm_kernel<<<dim3(static_cast<unsigned>(rb.nblocks)), dim3(kBS), 0, as_stream(stream)>>>(
+        P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(), rb.desc.as<int2>(), rb.nblocks, alpha,
         u, v, out);
     check_launch("sddmm_kernel");
   });

@@ -1,0 +1,93 @@
+"""GPU checks of the adjoint (extension 3): theta -> A assembly, SDDMM
+gradient, torch autograd against the CPU oracle's adjoint, central
+differences and torch.autograd.gradcheck."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import spmv_amd as S
+from spmv_amd import adjoint as AD
+from spmv_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def problem(n, seed=1):
+    ne = P.diffusion_num_edges(n)
+    theta = np.exp(0.5 * P.normal(seed, ne))       # lognormal(0, 0.5)
+    b = P.uniform(seed + 1, n * n, 0.0, 1.0)
+    xt = P.normal(seed + 2, n * n)
+    return theta, b, xt
+
+
+def test_assembly_bitexact():
+    n = 37
+    theta, _, _ = problem(n)
+    op = AD.DiffusionOperator(n)
+    vals = op.assemble(torch.from_numpy(theta).to(DEV)).cpu().numpy()
+    assert np.array_equal(vals, O.gen_diffusion(n, theta).val)
+
+
+def test_gradients_match_oracle_adjoint():
+    n = 48
+    theta, b, xt = problem(n, 5)
+    op = AD.DiffusionOperator(n)
+    th = torch.from_numpy(theta).to(DEV).requires_grad_(True)
+    bd = torch.from_numpy(b).to(DEV).requires_grad_(True)
+    x = op.solve(th, bd, rtol=1e-14)
+    loss = 0.5 * ((x - torch.from_numpy(xt).to(DEV)) ** 2).sum()
+    loss.backward()
+    ref = O.adjoint_diffusion(n, theta, b, xt, 100000, 1e-14)
+    assert abs(loss.item() - ref["loss"]) <= 1e-10 * abs(ref["loss"])
+    gt = th.grad.cpu().numpy()
+    assert np.max(np.abs(gt - ref["grad_theta"])) <= 1e-9 * np.max(np.abs(ref["grad_theta"]))
+    assert np.max(np.abs(bd.grad.cpu().numpy() - ref["grad_b"])) <= 1e-9 * np.max(np.abs(ref["grad_b"]))
+
+
+def test_sddmm_gradient_is_minus_lambda_x():
+    n = 20
+    theta, b, xt = problem(n, 9)
+    op = AD.DiffusionOperator(n)
+    vals = op.assemble(torch.from_numpy(theta).to(DEV)).requires_grad_(True)
+    x = AD.SparseSolve.apply(vals, torch.from_numpy(b).to(DEV), op.A, 100000, 1e-14)
+    loss = 0.5 * ((x - torch.from_numpy(xt).to(DEV)) ** 2).sum()
+    loss.backward()
+    A = O.gen_diffusion(n, theta).to_scipy().toarray()
+    xs = np.linalg.solve(A, b)
+    lam = np.linalg.solve(A.T, xs - xt)
+    pat = op.pattern
+    rows = np.repeat(np.arange(pat.nrows), np.diff(pat.rowptr))
+    want = -lam[rows] * xs[pat.colidx]
+    assert np.allclose(vals.grad.cpu().numpy(), want, rtol=0, atol=1e-10 * np.abs(want).max())
+
+
+def test_central_differences_32_entries():
+    n = 64
+    theta, b, xt = problem(n, 13)
+    op = AD.DiffusionOperator(n)
+    th = torch.from_numpy(theta).to(DEV).requires_grad_(True)
+    bd = torch.from_numpy(b).to(DEV)
+    xtd = torch.from_numpy(xt).to(DEV)
+    x = op.solve(th, bd, rtol=1e-14)
+    (0.5 * ((x - xtd) ** 2).sum()).backward()
+    entries = (P.rand_u64(77, 32) % np.uint64(op.num_edges)).astype(np.int64)
+    fd = AD.central_difference(op, th.detach(), bd, xtd, entries, eps=1e-6, rtol=1e-14)
+    g = th.grad.cpu().numpy()[entries]
+    assert np.max(np.abs(fd - g) / np.maximum(np.abs(g), 1e-300)) <= 1e-6
+
+
+def test_gradcheck_tiny():
+    n = 4
+    theta, b, xt = problem(n, 21)
+    op = AD.DiffusionOperator(n)
+    xtd = torch.from_numpy(xt).to(DEV)
+
+    def f(th, bb):
+        x = op.solve(th, bb, rtol=1e-15)
+        return 0.5 * ((x - xtd) ** 2).sum()
+
+    th = torch.from_numpy(theta).to(DEV).requires_grad_(True)
+    bd = torch.from_numpy(b).to(DEV).requires_grad_(True)
+    assert torch.autograd.gradcheck(f, (th, bd), eps=1e-6, atol=1e-6, rtol=1e-5)

@@ -1,0 +1,103 @@
+"""GPU checks of the batched persistent CG (extension 2) against the oracle's
+FP32 CG (the loop of spmv/cg.cpp:50-80 in single precision), per system and
+right-hand side."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import spmv_amd as S
+from spmv_amd import autotune as T
+from spmv_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def make_systems(sizes, seed):
+    return [P.random_spd(int(n), seed + i) for i, n in enumerate(sizes)]
+
+
+def rhs(nrows, nrhs, seed):
+    return P.uniform(seed, nrows * nrhs, -1, 1).astype(np.float32).reshape(nrows, nrhs)
+
+
+@pytest.mark.parametrize("nrhs", [4, 2, 1])
+def test_batch_matches_oracle(nrhs):
+    systems = make_systems([1500, 700, 3000, 2200, 999, 4100, 1800, 640], 100)
+    B = S.Batch(systems, nrhs=nrhs)
+    b = rhs(B.nrows, nrhs, 7)
+    x, its = B.solve(torch.from_numpy(b).to(DEV), max_its=2000, rtol=1e-6)
+    xh = x.cpu().numpy()
+    for s, A in enumerate(systems):
+        lo, hi = B.sys_row[s], B.sys_row[s + 1]
+        oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
+        for j in range(nrhs):
+            xo, ko, _ = O.cg_f32(oc, A.val.astype(np.float32), b[lo:hi, j], 2000, 1e-6)
+            assert abs(int(its[s, j]) - ko) <= 2, (s, j, its[s, j], ko)
+            bn = np.linalg.norm(b[lo:hi, j])
+            r = b[lo:hi, j].astype(np.float64) - O.spmv(oc, xh[lo:hi, j].astype(np.float64))
+            assert np.linalg.norm(r) <= 5e-6 * bn
+            assert np.linalg.norm(xh[lo:hi, j] - xo) <= 1e-4 * np.linalg.norm(xo)
+
+
+def true_rel_residuals(systems, B, b, x):
+    out = []
+    for s, A in enumerate(systems):
+        lo, hi = B.sys_row[s], B.sys_row[s + 1]
+        oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
+        for j in range(b.shape[1]):
+            r = b[lo:hi, j].astype(np.float64) - O.spmv(oc, x[lo:hi, j].astype(np.float64))
+            out.append(np.linalg.norm(r) / np.linalg.norm(b[lo:hi, j]))
+    return np.array(out)
+
+
+def test_batch_every_config_and_replay():
+    """Every (workgroup size, lanes per row) pair converges in the same
+    number of iterations (+-2; the reduction order differs) to a solution
+    whose true residual meets the tolerance."""
+    systems = make_systems([900, 2600, 1300, 300, 4000], 300)
+    B = S.Batch(systems, nrhs=4)
+    bh = rhs(B.nrows, 4, 3)
+    bd = torch.from_numpy(bh).to(DEV)
+    ref, its0 = B.solve(bd, max_its=1000, rtol=1e-6)
+    for bs in T.BATCH_BLOCKS:
+        for g in T.BATCH_LANES:
+            B.set_config(bs, g)
+            x, its = B.solve(bd, max_its=1000, rtol=1e-6)
+            assert np.max(np.abs(its - its0)) <= 2
+            assert np.max(true_rel_residuals(systems, B, bh, x.cpu().numpy())) <= 5e-6
+    # graph replays: same answer every time, with eager work in between
+    B.set_config(256, 0)
+    out = torch.empty_like(bd)
+    first, _ = B.solve(bd, out, max_its=1000, rtol=1e-6)
+    first = first.clone()
+    for _ in range(3):
+        torch.cuda.synchronize()
+        _ = (bd * 2.0).sum().item()
+        x, _ = B.solve(bd, out, max_its=1000, rtol=1e-6)
+        assert torch.equal(x, first)
+
+
+def test_batch_max_its_and_zero_rhs():
+    systems = make_systems([800, 1200], 50)
+    B = S.Batch(systems, nrhs=4)
+    b = rhs(B.nrows, 4, 1)
+    b[: systems[0].nrows, 1] = 0.0
+    x, its = B.solve(torch.from_numpy(b).to(DEV), max_its=3, rtol=1e-12)
+    assert np.all(its[:, [0, 2, 3]] == 3)
+    x0, its0 = B.solve(torch.from_numpy(b).to(DEV), max_its=0, rtol=1e-6)
+    assert np.all(its0 == 0) and torch.all(x0 == 0)
+
+
+def test_autotuner_caches(tmp_path, monkeypatch):
+    monkeypatch.setenv("SPMV_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    systems = make_systems([2000, 1000, 500], 9)
+    B = S.Batch(systems, nrhs=4)
+    bd = torch.from_numpy(rhs(B.nrows, 4, 2)).to(DEV)
+    best, table = T.tune_batch(B, bd, max_its=500, rtol=1e-6, reps=1)
+    assert table is not None and len(table) == len(T.BATCH_BLOCKS) * len(T.BATCH_LANES)
+    cfg = B.config()
+    assert (cfg["block"], cfg["lanes_per_row"]) == (best["block"], best["lanes_per_row"])
+    best2, table2 = T.tune_batch(B, bd, max_its=500, rtol=1e-6, reps=1)
+    assert table2 is None and best2 == best

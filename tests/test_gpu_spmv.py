@@ -181,3 +181,21 @@ def test_sddmm():
     g = S.sddmm(M, _dev(u), _dev(v), alpha=-1.0).cpu().numpy()
     rows = np.repeat(np.arange(A.nrows), np.diff(A.rowptr))
     assert np.array_equal(g, (-u[rows]) * v[A.colidx])
+
+
+@pytest.mark.parametrize("name,A", [MATS[3], MATS[5], MATS[6]], ids=["stencil27_ghost", "random_spd", "long_rows"])
+def test_all_kernel_variants_bitexact(name, A):
+    want = O.spmv(_ocsr(A), P.uniform(7, A.ncols, -1, 1))
+    x = _dev(P.uniform(7, A.ncols, -1, 1))
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    Mb = M.like("bf16")
+    xf = x.float()
+    ref_b = None
+    for v in range(S.lib().spmvx_num_kernels()):
+        M.set_kernel(v)
+        assert np.array_equal(M.mult(x).cpu().numpy(), want), v
+        Mb.set_kernel(v)
+        yb = Mb.mult(xf).cpu().numpy()
+        if ref_b is None:
+            ref_b = yb
+        assert np.array_equal(yb, ref_b), v
