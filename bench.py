@@ -102,6 +102,94 @@ def run_cg_section(torch, S, P, O, n, seed, cpu_iters):
     return out
 
 
+def run_batch_section(torch, S, P, nsys=256, nrhs=4, rtol=1e-6, seed=17):
+    """256 SPD systems (rows log-uniform in [1e3, 2e5], power-law row
+    lengths in [3, 64], diagonal 1.1 x |off-diagonal row sum|, FP32), 4 RHS
+    each, one persistent hipGraph-captured CG launch to rtol."""
+    import numpy as np
+    from spmv_amd import autotune as T
+    t0 = time.time()
+    ns = np.exp(np.log(1e3) + (np.log(2e5) - np.log(1e3)) * P.uniform(seed, nsys)).astype(np.int64)
+    systems = [P.random_spd(int(n), 1000 + i) for i, n in enumerate(ns)]
+    B = S.Batch(systems, nrhs=nrhs)
+    log(f"[batch] {nsys} systems, {B.nrows} rows, {B.nnz} nnz, generated in {time.time() - t0:.1f}s")
+    b = torch.from_numpy(P.uniform(23, B.nrows * nrhs, -1, 1).astype(np.float32)
+                         .reshape(B.nrows, nrhs)).to("cuda:0")
+    best, table = T.tune_batch(B, b, max_its=20000, rtol=rtol, reps=1)
+    x, its = B.solve(b, max_its=20000, rtol=rtol)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    B.solve(b, x, 20000, rtol, iterations=False)
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t
+    # true residuals of a sample of systems (CPU, float64)
+    import oracle as O
+    worst = 0.0
+    xh, bh = x.cpu().numpy(), b.cpu().numpy()
+    for s_ in list(range(0, nsys, 32)):
+        A = systems[s_]
+        lo, hi = B.sys_row[s_], B.sys_row[s_ + 1]
+        oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
+        for j in range(nrhs):
+            r = bh[lo:hi, j].astype(np.float64) - O.spmv(oc, xh[lo:hi, j].astype(np.float64))
+            worst = max(worst, float(np.linalg.norm(r) / np.linalg.norm(bh[lo:hi, j])))
+    out = {"workload": f"{nsys} random SPD systems x {nrhs} RHS, {B.nrows} rows, {B.nnz} nnz, FP32, rtol {rtol}",
+           "seconds": ts, "systems_per_s": nsys / ts, "rhs_solves_per_s": nsys * nrhs / ts,
+           "iterations_max": int(its.max()), "iterations_mean": float(its.mean()),
+           "config": B.config(), "tuned": best, "worst_true_rel_residual_sampled": worst}
+    log(f"[batch] {ts:.3f}s its max {its.max()} mean {its.mean():.1f} config {B.config()}")
+    del B
+    return out
+
+
+def run_adjoint_section(torch, S, P, fd_entries, n=1024, rtol=1e-13, seed=1):
+    """2D anisotropic diffusion 1024^2, theta ~ lognormal(0, 0.5), b ~ U[0,1],
+    x_target ~ N(0,1); loss 0.5||x - x_t||^2; forward CG, adjoint CG + SDDMM
+    through torch autograd; central differences (eps 1e-6) on seeded
+    entries."""
+    import numpy as np
+    from spmv_amd import adjoint as AD
+    dev = "cuda:0"
+    ne = P.diffusion_num_edges(n)
+    theta = np.exp(0.5 * P.normal(seed, ne))
+    b = P.uniform(seed + 1, n * n, 0.0, 1.0)
+    xt = P.normal(seed + 2, n * n)
+    op = AD.DiffusionOperator(n)
+    th = torch.from_numpy(theta).to(dev).requires_grad_(True)
+    bd = torch.from_numpy(b).to(dev).requires_grad_(True)
+    xtd = torch.from_numpy(xt).to(dev)
+    op.solve(th.detach(), bd.detach(), 2, rtol)  # warm-up (workspaces)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    x = op.solve(th, bd, rtol=rtol)
+    loss = 0.5 * ((x - xtd) ** 2).sum()
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t
+    t = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t
+    # SDDMM kernel alone (grad_A = -lam x^T on the pattern)
+    lam = bd.grad.detach()
+    g = torch.empty(op.A.nnz, dtype=torch.float64, device=dev)
+    sd_ms = timed_loop(torch, lambda: S.sddmm(op.A, lam, x.detach(), alpha=-1.0, out=g), 20)
+    sd_bytes = op.A.nnz * (4 + 8) + 4 * (op.A.nrows + 1) + 2 * 8 * op.A.nrows
+    out = {"workload": f"anisotropic 5-point diffusion {n}^2, {op.A.nnz} nnz, {ne} edge conductivities",
+           "forward_seconds": tf, "backward_seconds": tb, "rtol": rtol, "loss": loss.item(),
+           "sddmm_ms": sd_ms, "sddmm_GBps": sd_bytes / sd_ms / 1e6}
+    log(f"[adjoint] forward {tf:.3f}s backward {tb:.3f}s sddmm {sd_ms:.3f} ms")
+    if fd_entries > 0:
+        entries = (P.rand_u64(77, fd_entries) % np.uint64(ne)).astype(np.int64)
+        t = time.perf_counter()
+        fd = AD.central_difference(op, th.detach(), bd.detach(), xtd, entries, eps=1e-6, rtol=rtol)
+        gt = th.grad.cpu().numpy()[entries]
+        rel = np.abs(fd - gt) / np.maximum(np.abs(gt), 1e-300)
+        out.update({"fd_entries": int(fd_entries), "fd_eps": 1e-6, "fd_max_rel_err": float(rel.max()),
+                    "fd_median_rel_err": float(np.median(rel)), "fd_seconds": time.perf_counter() - t})
+        log(f"[adjoint] FD on {fd_entries} entries: max rel err {rel.max():.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +199,9 @@ def main():
     ap.add_argument("--cg-n", type=int, default=256, help="grid edge of the CG comparison")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--fd-entries", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--mode", choices=["bench", "spmv", "copy"], default="bench",
                     help="spmv: only the SpMV kernel loop; copy: the calibration copy kernel")
@@ -255,6 +346,10 @@ def main():
         import oracle as O
         result["cg"] = run_cg_section(torch, S, P, O, args.cg_n, 5,
                                       0 if args.no_cpu else 3)
+    if not args.no_batch:
+        result["batch"] = run_batch_section(torch, S, P)
+    if not args.no_adjoint:
+        result["adjoint"] = run_adjoint_section(torch, S, P, args.fd_entries)
     print(json.dumps(result))
 
 

@@ -208,20 +208,14 @@ typedef struct
   int32_t inner_hist[64]; /* inner iterations of step j                       */
 } spmvx_ir_stats;
 
-/* spmvx_ir_cg flags */
-#define SPMVX_IR_CARRY_DIRECTION 1
-
 /*
  * Solves A64 x = b to ||b - A64 x|| / ||b|| <= rtol.  Each outer step forms
  * r = b - A64 x (FP64 SpMV) and ||r|| in FP64 on the device, runs CG in FP32
  * on Alow d = r / ||r|| (Alow: SPMVX_F32 or SPMVX_BF16 copy with the same
  * pattern; FP32 vectors and accumulation; stop at relative residual
  * inner_rtol or inner_max_its) and updates x += ||r|| d in FP64.
- * flags = 0: every inner solve restarts from d = 0, p = r (classic
- * refinement).  SPMVX_IR_CARRY_DIRECTION: the inner solve keeps its previous
- * search direction, rescaled to the new normalisation, and only replaces the
- * residual by the FP64 one (residual replacement / "reliable updates"): the
- * Krylov information survives the outer steps.
+ * Every inner solve starts from d = 0, p = r (classic refinement).  flags
+ * is reserved and must be 0.
  * x_dev[ncols of A64]; ghost columns are treated as zero.
  */
 int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b_dev, double* x_dev,

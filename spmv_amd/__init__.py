@@ -404,10 +404,8 @@ def cg_f32(A, b, max_its, rtol, stream=None):
 
 
 def ir_cg(A64, Alow, b, rtol=1e-10, inner_rtol=1e-4, max_outer=32, inner_max_its=100000,
-          carry_direction=False, stream=None):
-    """Mixed-precision iterative refinement: returns (x, stats dict).
-    carry_direction keeps the inner CG's search direction across outer steps
-    (residual replacement) instead of restarting it."""
+          stream=None):
+    """Mixed-precision iterative refinement: returns (x, stats dict)."""
     import torch
     if b.numel() != A64.nrows or b.dtype != torch.float64:
         raise SpmvError("b must be float64 with A.nrows entries")
@@ -415,7 +413,7 @@ def ir_cg(A64, Alow, b, rtol=1e-10, inner_rtol=1e-4, max_outer=32, inner_max_its
     st = IRStats()
     _check(lib().spmvx_ir_cg(A64.handle, Alow.handle, _dptr(b), _dptr(x), int(max_outer),
                              float(rtol), int(inner_max_its), float(inner_rtol),
-                             1 if carry_direction else 0, ctypes.byref(st), _stream(stream)))
+                             0, ctypes.byref(st), _stream(stream)))
     return x, st.as_dict()
 
 

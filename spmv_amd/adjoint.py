@@ -45,9 +45,13 @@ class SparseSolve(torch.autograd.Function):
     def backward(ctx, grad_x):
         (x,) = ctx.saved_tensors
         A = ctx.A
-        # A symmetric: A^T lam = grad_x is solved with the same CG
-        lam, its = cg(A, grad_x.contiguous(), ctx.max_its, ctx.rtol)
-        ctx.its.append(its)
+        # A symmetric: A^T lam = grad_x is solved with the same CG.  A zero
+        # seed has the exact adjoint lam = 0 (CG itself would form 0/0).
+        if not torch.any(grad_x):
+            lam = torch.zeros_like(x)
+        else:
+            lam, its = cg(A, grad_x.contiguous(), ctx.max_its, ctx.rtol)
+            ctx.its.append(its)
         grad_values = sddmm(A, lam, x, alpha=-1.0) if ctx.needs_input_grad[0] else None
         grad_b = lam if ctx.needs_input_grad[1] else None
         return grad_values, grad_b, None, None, None

@@ -70,8 +70,7 @@ __global__ void cg_state_init(CgState<Acc>* st, int32_t kmax, Acc rtol2)
 // Host driver of one CG solve.  x has ncols entries; b has nrows.
 template <typename V, typename X, typename Acc>
 int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t vec_dtype,
-           const X* b, X* x, int32_t kmax, Acc rtol, Acc* hist_dev, hipStream_t s,
-           X carry = X(0))
+           const X* b, X* x, int32_t kmax, Acc rtol, Acc* hist_dev, hipStream_t s)
 {
   Pattern& P = *A->pat;
   const RowBlocks& rb = P.blocks_for(variant_block_nnz(A->variant));
@@ -87,7 +86,7 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
   const int vg = vec_grid(n);
   const int g0 = vec_grid(ncols > n ? ncols : n);
   cg_state_init<Acc><<<1, 1, 0, s>>>(st, kmax, rtol * rtol);
-  cg_init<X, Acc><<<g0, kVecBS, 0, s>>>(n, ncols, b, x, r, p, part_0, carry);
+  cg_init<X, Acc><<<g0, kVecBS, 0, s>>>(n, ncols, b, x, r, p, part_0);
   cg_init_finalize<Acc><<<1, kFinBS, 0, s>>>(part_0, g0, st, hist_dev);
   check_launch("cg_init");
   auto* flag = static_cast<CgState<Acc>*>(A->host_flag.ensure(sizeof(CgState<Acc>)));
@@ -244,8 +243,8 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
     zero_f64<<<vg, kVecBS, 0, s>>>(x, n);
     spmvx_ir_stats local{};
     auto* flag = static_cast<CgState<double>*>(A64->host_flag.ensure(sizeof(CgState<double>)));
-    double bnorm = 0.0, rn_prev = 0.0;
-    const bool carry_dir = (flags & SPMVX_IR_CARRY_DIRECTION) != 0;
+    double bnorm = 0.0;
+    SPMVX_REQUIRE(flags == 0, "no refinement flags are defined");
     int32_t outer = 0;
     for (;;)
     {
@@ -270,19 +269,13 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
       // inner FP32 solve on the normalised residual
       scale_to_f32<<<vg, kVecBS, 0, s>>>(r, r32, n, st);
       check_launch("scale_to_f32");
-      // carry: keep the inner search direction, rescaled from the previous
-      // normalisation (1/rn_prev) to this one (1/rn)
-      const float carry = (carry_dir && outer > 0) ? static_cast<float>(rn_prev / rn) : 0.0f;
-      rn_prev = rn;
       int k;
       if (Alow->dtype == SPMVX_F32)
         k = run_cg<float, float, float>(Alow, Alow->ws32, nullptr, SPMVX_F32, r32, d32,
-                                        inner_max_its, static_cast<float>(inner_rtol), nullptr, s,
-                                        carry);
+                                        inner_max_its, static_cast<float>(inner_rtol), nullptr, s);
       else
         k = run_cg<bf16_t, float, float>(Alow, Alow->ws32, nullptr, SPMVX_F32, r32, d32,
-                                         inner_max_its, static_cast<float>(inner_rtol), nullptr, s,
-                                         carry);
+                                         inner_max_its, static_cast<float>(inner_rtol), nullptr, s);
       local.inner_hist[outer] = k;
       local.inner_its += k;
       add_correction<<<vg, kVecBS, 0, s>>>(x, d32, n, st);

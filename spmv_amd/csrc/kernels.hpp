@@ -578,14 +578,12 @@ int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t
 // ---------------------------------------------------------------- CG vectors
 constexpr int kVecBS = 256;
 
-// x = 0, r = b, p[0:n) = b, p[n:ncols) = 0, partial ||b||^2 (cg.cpp:31-44).
-// With carry > 0 the previous direction is kept, rescaled (p = carry * p):
-// the residual-replacement restart of the refinement loop.
+// x = 0, r = b, p[0:n) = b, p[n:ncols) = 0, partial ||b||^2 (cg.cpp:31-44)
 template <typename X, typename Acc>
 __global__ __launch_bounds__(kVecBS) void cg_init(int64_t n, int64_t ncols,
                                                   const X* __restrict__ b, X* __restrict__ x,
                                                   X* __restrict__ r, X* __restrict__ p,
-                                                  Acc* __restrict__ partials, X carry)
+                                                  Acc* __restrict__ partials)
 {
   __shared__ Acc red[16];
   Acc part = Acc(0);
@@ -597,7 +595,7 @@ __global__ __launch_bounds__(kVecBS) void cg_init(int64_t n, int64_t ncols,
     {
       const X bi = b[i];
       r[i] = bi;
-      p[i] = carry > X(0) ? carry * p[i] : bi;
+      p[i] = bi;
       part += static_cast<Acc>(bi) * static_cast<Acc>(bi);
     }
     else
