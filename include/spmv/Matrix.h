@@ -1,0 +1,44 @@
+// spmv::Matrix<T> -- distributed CSR matrix block (the interface of
+// spmv/Matrix.h:23-81) stored and multiplied on the device, backed by the
+// spmvx C ABI.  T = double (FP64) or float (FP32).
+#pragma once
+
+#include <memory>
+
+#include "L2GMap.h"
+#include "Vector.h"
+
+namespace spmv
+{
+
+template <typename T>
+class Matrix
+{
+public:
+  /// Copies the CSR in (Matrix.cpp:13-22) and runs the row-binning analysis.
+  Matrix(const CSR<T>& A, std::shared_ptr<L2GMap> col_map, std::shared_ptr<L2GMap> row_map);
+  ~Matrix();
+  Matrix(Matrix&& o) noexcept;
+  Matrix(const Matrix&) = delete;
+  Matrix& operator=(const Matrix&) = delete;
+
+  /// y = A x (Matrix.h:37-39): x has cols() entries (owned + ghosts).
+  Vector<T> operator*(const Vector<T>& b) const;
+  /// y = A^T x (Matrix.h:41-43): x has rows() entries.
+  Vector<T> transpmult(const Vector<T>& b) const;
+
+  std::shared_ptr<const L2GMap> row_map() const { return _row_map; }
+  std::shared_ptr<const L2GMap> col_map() const { return _col_map; }
+  std::int64_t rows() const { return _rows; }
+  std::int64_t cols() const { return _cols; }
+  std::int64_t nonZeros() const { return _nnz; }
+
+  spmvx_matrix handle() const { return _h; }
+
+private:
+  spmvx_matrix _h = nullptr;
+  std::int64_t _rows = 0, _cols = 0, _nnz = 0;
+  std::shared_ptr<L2GMap> _col_map, _row_map;
+};
+
+} // namespace spmv

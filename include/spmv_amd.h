@@ -241,22 +241,28 @@ int spmvx_batch_create(int32_t nsys, const int32_t* sys_row_host, const int32_t*
                        const int32_t* colidx_host, const float* values_host, int32_t nrhs,
                        spmvx_batch* out);
 int spmvx_batch_destroy(spmvx_batch B);
-/* Launch configuration: workgroup size (256, 512, 1024) and lanes per row
- * (1, 2, 4, 8, 16; 0 = chosen per system from its mean row length).  Set by
+/* Launch configuration: workgroup size (256, 512, 1024), lanes per row
+ * (1, 2, 4, 8, 16; 0 = chosen per system from its mean row length) and rows
+ * per team member: a system of n rows is solved by a team of
+ * ceil(n / rows_per_member) workgroups (capped at 64 and at half the
+ * workgroups resident on the device; 0 = one workgroup per system).  Set by
  * the autotuner (spmv_amd/autotune.py). */
-int spmvx_batch_set_config(spmvx_batch B, int32_t block, int32_t lanes_per_row);
+int spmvx_batch_set_config(spmvx_batch B, int32_t block, int32_t lanes_per_row,
+                           int32_t rows_per_member);
 /*
  * Solves every system and right-hand side with CG from x = 0 until
  * ||r_k||^2 / ||b||^2 < rtol^2 or max_its, in one persistent kernel launch
- * (one workgroup per system, systems taken largest first from a device work
- * queue), replayed from a hipGraph captured on the first call with these
- * arguments.  its_host (may be NULL): nsys*nrhs iteration counts; when given
- * the stream is synchronised.
+ * (workgroups take team slots, largest systems first, from a device work
+ * queue; team members synchronise through bounded waits), replayed from a
+ * hipGraph captured on the first call with these arguments.  its_host (may be
+ * NULL): nsys*nrhs iteration counts; when given the stream is synchronised and
+ * a timed-out team wait is reported as SPMVX_ERR_STATE.
  */
 int spmvx_batch_solve(spmvx_batch B, const float* b_dev, float* x_dev, int32_t max_its,
                       double rtol, int32_t* its_host, spmvx_stream stream);
 int spmvx_batch_info(spmvx_batch B, int32_t* nsys, int64_t* nrows, int64_t* nnz, int32_t* nrhs,
-                     int32_t* block, int32_t* lanes_per_row, int32_t* grid);
+                     int32_t* block, int32_t* lanes_per_row, int32_t* rows_per_member,
+                     int32_t* grid, int32_t* nslots);
 
 /* ===================================================================== */
 /* Extension 3: adjoint helpers (no reference counterpart)                */

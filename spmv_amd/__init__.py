@@ -91,9 +91,9 @@ def lib():
         "spmvx_copy_f64": ([P, P, i64, P], ctypes.c_int),
         "spmvx_batch_create": ([i32, P, P, P, P, i32, P], ctypes.c_int),
         "spmvx_batch_destroy": ([P], ctypes.c_int),
-        "spmvx_batch_set_config": ([P, i32, i32], ctypes.c_int),
+        "spmvx_batch_set_config": ([P, i32, i32, i32], ctypes.c_int),
         "spmvx_batch_solve": ([P, P, P, i32, f64, P, P], ctypes.c_int),
-        "spmvx_batch_info": ([P, P, P, P, P, P, P, P], ctypes.c_int),
+        "spmvx_batch_info": ([P, P, P, P, P, P, P, P, P, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -471,14 +471,19 @@ class Batch:
             _LIB.spmvx_batch_destroy(self._h)
             self._h = None
 
-    def set_config(self, block, lanes_per_row):
-        _check(lib().spmvx_batch_set_config(self._h, int(block), int(lanes_per_row)))
+    def set_config(self, block, lanes_per_row, rows_per_member=None):
+        if rows_per_member is None:
+            rows_per_member = self.config()["rows_per_member"]
+        _check(lib().spmvx_batch_set_config(self._h, int(block), int(lanes_per_row),
+                                            int(rows_per_member)))
 
     def config(self):
         v = [ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32(),
-             ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()]
+             ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(),
+             ctypes.c_int32()]
         _check(lib().spmvx_batch_info(self._h, *[ctypes.byref(x) for x in v]))
-        return dict(block=v[4].value, lanes_per_row=v[5].value, grid=v[6].value)
+        return dict(block=v[4].value, lanes_per_row=v[5].value, rows_per_member=v[6].value,
+                    grid=v[7].value, slots=v[8].value)
 
     def solve(self, b, x=None, max_its=10000, rtol=1e-6, iterations=True, stream=None):
         """Returns x (and the (nsys, nrhs) iteration counts when iterations)."""

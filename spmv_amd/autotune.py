@@ -16,7 +16,8 @@ import os
 import numpy as np
 
 BATCH_BLOCKS = (256, 512, 1024)
-BATCH_LANES = (0, 1, 2, 4, 8, 16)
+BATCH_LANES = (0, 2, 4, 8)
+BATCH_ROWS_PER_MEMBER = (4096, 16384, 65536)
 
 
 def cache_path():
@@ -69,26 +70,27 @@ def batch_signature(batch, max_its, rtol):
 
 
 def tune_batch(batch, b, max_its=10000, rtol=1e-6, reps=2, blocks=BATCH_BLOCKS,
-               lanes=BATCH_LANES, use_cache=True):
-    """Times every (workgroup size, lanes per row) pair on the batch solve,
-    applies the fastest and returns (best, table)."""
+               lanes=BATCH_LANES, rows_per_member=BATCH_ROWS_PER_MEMBER, use_cache=True):
+    """Times every (workgroup size, lanes per row, rows per team member)
+    triple on the batch solve, applies the fastest and returns (best, table)."""
     import torch
     key = batch_signature(batch, max_its, rtol)
     if use_cache:
         hit = _load().get(key)
         if hit:
-            batch.set_config(hit["block"], hit["lanes_per_row"])
+            batch.set_config(hit["block"], hit["lanes_per_row"], hit["rows_per_member"])
             return hit, None
     x = torch.empty_like(b)
     table = []
     for bs in blocks:
         for g in lanes:
-            batch.set_config(bs, g)
-            batch.solve(b, x, max_its, rtol, iterations=False)  # capture + warm up
-            ms = _time_ms(lambda: batch.solve(b, x, max_its, rtol, iterations=False), reps)
-            table.append({"block": bs, "lanes_per_row": g, "ms": ms})
+            for rpm in rows_per_member:
+                batch.set_config(bs, g, rpm)
+                batch.solve(b, x, max_its, rtol, iterations=False)  # capture + warm up
+                ms = _time_ms(lambda: batch.solve(b, x, max_its, rtol, iterations=False), reps)
+                table.append({"block": bs, "lanes_per_row": g, "rows_per_member": rpm, "ms": ms})
     best = min(table, key=lambda r: r["ms"])
-    batch.set_config(best["block"], best["lanes_per_row"])
+    batch.set_config(best["block"], best["lanes_per_row"], best["rows_per_member"])
     _store(key, best)
     return best, table
 

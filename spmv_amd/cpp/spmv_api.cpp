@@ -1,0 +1,316 @@
+// C++ API (spmv::Vector, spmv::L2GMap, spmv::Matrix, spmv::cg) over the
+// spmvx C ABI.  Errors become std::runtime_error carrying the library's
+// message, which for the reference's checks is the reference's text.
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+#include "spmv/spmv.h"
+
+namespace spmv
+{
+namespace
+{
+
+void check(int status)
+{
+  if (status != SPMVX_OK)
+    throw std::runtime_error(spmvx_last_error());
+}
+
+void hip_check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+constexpr int32_t dtype_of()
+{
+  return std::is_same<T, double>::value ? SPMVX_F64 : SPMVX_F32;
+}
+
+int comm_size(MPI_Comm comm)
+{
+  int size = 1;
+  MPI_Comm_size(comm, &size);
+  return size;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------- Vector
+template <typename T>
+Vector<T>::Vector(std::size_t n) : _n(n)
+{
+  if (n)
+    hip_check(hipMalloc(&_p, n * sizeof(T)), "hipMalloc");
+  setZero();
+}
+
+template <typename T>
+Vector<T>::Vector(const std::vector<T>& host) : Vector(host.size())
+{
+  from_host(host);
+}
+
+template <typename T>
+Vector<T>::~Vector()
+{
+  if (_p)
+    (void)hipFree(_p);
+}
+
+template <typename T>
+Vector<T>::Vector(Vector&& o) noexcept : _p(o._p), _n(o._n)
+{
+  o._p = nullptr;
+  o._n = 0;
+}
+
+template <typename T>
+Vector<T>& Vector<T>::operator=(Vector&& o) noexcept
+{
+  if (this != &o)
+  {
+    if (_p)
+      (void)hipFree(_p);
+    _p = o._p;
+    _n = o._n;
+    o._p = nullptr;
+    o._n = 0;
+  }
+  return *this;
+}
+
+template <typename T>
+Vector<T> Vector<T>::clone() const
+{
+  Vector<T> v(_n);
+  if (_n)
+    hip_check(hipMemcpy(v._p, _p, _n * sizeof(T), hipMemcpyDeviceToDevice), "hipMemcpy");
+  return v;
+}
+
+template <typename T>
+void Vector<T>::setZero()
+{
+  if (_n)
+    hip_check(hipMemset(_p, 0, _n * sizeof(T)), "hipMemset");
+}
+
+template <typename T>
+void Vector<T>::copy_head(const Vector& src, std::size_t n)
+{
+  if (n > _n || n > src._n)
+    throw std::runtime_error("copy_head: vectors too short");
+  if (n)
+    hip_check(hipMemcpy(_p, src._p, n * sizeof(T), hipMemcpyDeviceToDevice), "hipMemcpy");
+}
+
+template <typename T>
+std::vector<T> Vector<T>::to_host() const
+{
+  std::vector<T> h(_n);
+  if (_n)
+    hip_check(hipMemcpy(h.data(), _p, _n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy");
+  return h;
+}
+
+template <typename T>
+void Vector<T>::from_host(const std::vector<T>& host)
+{
+  if (host.size() != _n)
+    throw std::runtime_error("from_host: size mismatch");
+  if (_n)
+    hip_check(hipMemcpy(_p, host.data(), _n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+template class Vector<double>;
+template class Vector<float>;
+
+// ---------------------------------------------------------------- L2GMap
+L2GMap::L2GMap(MPI_Comm comm, std::int64_t local_size, const std::vector<std::int64_t>& ghosts)
+    : _ghosts(ghosts)
+{
+  Decomposition d;
+  d.local_sizes = {local_size};
+  d.rank = 0;
+  build(comm, local_size, d);
+}
+
+L2GMap::L2GMap(MPI_Comm comm, std::int64_t local_size, const std::vector<std::int64_t>& ghosts,
+               const Decomposition& decomp)
+    : _ghosts(ghosts)
+{
+  build(comm, local_size, decomp);
+}
+
+void L2GMap::build(MPI_Comm comm, std::int64_t local_size, const Decomposition& d)
+{
+  if (comm_size(comm) != 1)
+    throw std::runtime_error("spmv::L2GMap: one process drives one rank here; a communicator "
+                             "with several processes needs the multi-GPU transport");
+  const int nranks = static_cast<int>(d.local_sizes.size());
+  if (nranks < 1 || d.rank < 0 || d.rank >= nranks || d.local_sizes[d.rank] != local_size)
+    throw std::runtime_error("spmv::L2GMap: inconsistent decomposition");
+  std::vector<std::int64_t> rptr(nranks + 1, 0), ridx;
+  for (int q = 0; q < nranks; ++q)
+  {
+    if (q < static_cast<int>(d.remote_ghosts.size()))
+      ridx.insert(ridx.end(), d.remote_ghosts[q].begin(), d.remote_ghosts[q].end());
+    rptr[q + 1] = static_cast<std::int64_t>(ridx.size());
+  }
+  check(spmvx_l2gmap_create(nranks, d.rank, d.local_sizes.data(), _ghosts.data(),
+                            static_cast<std::int64_t>(_ghosts.size()), rptr.data(),
+                            ridx.empty() ? nullptr : ridx.data(), &_h));
+}
+
+L2GMap::~L2GMap()
+{
+  if (_h)
+    spmvx_l2gmap_destroy(_h);
+}
+
+std::int32_t L2GMap::local_size() const
+{
+  std::int64_t v = 0;
+  check(spmvx_l2gmap_info(_h, &v, nullptr, nullptr, nullptr, nullptr, nullptr));
+  return static_cast<std::int32_t>(v);
+}
+
+std::int32_t L2GMap::num_ghosts() const { return static_cast<std::int32_t>(_ghosts.size()); }
+
+std::int64_t L2GMap::global_size() const
+{
+  std::int64_t v = 0;
+  check(spmvx_l2gmap_info(_h, nullptr, nullptr, &v, nullptr, nullptr, nullptr));
+  return v;
+}
+
+std::int64_t L2GMap::global_offset() const
+{
+  std::int64_t v = 0;
+  check(spmvx_l2gmap_info(_h, nullptr, nullptr, nullptr, &v, nullptr, nullptr));
+  return v;
+}
+
+std::int32_t L2GMap::global_to_local(std::int64_t i) const
+{
+  std::int32_t v = 0;
+  check(spmvx_l2gmap_global_to_local(_h, i, &v));
+  return v;
+}
+
+template <typename T>
+void L2GMap::update(T* vec_data) const
+{
+  check(spmvx_l2gmap_update(_h, dtype_of<T>(), vec_data, nullptr, nullptr));
+}
+
+template <typename T>
+void L2GMap::reverse_update(T* vec_data, const std::vector<T>& incoming) const
+{
+  std::int64_t nidx = 0;
+  check(spmvx_l2gmap_info(_h, nullptr, nullptr, nullptr, nullptr, &nidx, nullptr));
+  if (!incoming.empty() && static_cast<std::int64_t>(incoming.size()) != nidx)
+    throw std::runtime_error("reverse_update: incoming has the wrong length");
+  check(spmvx_l2gmap_reverse_update(_h, dtype_of<T>(), vec_data,
+                                    incoming.empty() ? nullptr : incoming.data(), nullptr,
+                                    nullptr));
+}
+
+template <typename T>
+void L2GMap::set_halo(const std::vector<T>& halo)
+{
+  if (halo.size() != _ghosts.size())
+    throw std::runtime_error("set_halo: length differs from num_ghosts");
+  check(spmvx_l2gmap_set_halo(_h, dtype_of<T>(), halo.data()));
+}
+
+void L2GMap::clear_halo() { check(spmvx_l2gmap_set_halo(_h, SPMVX_F64, nullptr)); }
+
+template void L2GMap::update<double>(double*) const;
+template void L2GMap::update<float>(float*) const;
+template void L2GMap::reverse_update<double>(double*, const std::vector<double>&) const;
+template void L2GMap::reverse_update<float>(float*, const std::vector<float>&) const;
+template void L2GMap::set_halo<double>(const std::vector<double>&);
+template void L2GMap::set_halo<float>(const std::vector<float>&);
+
+// ---------------------------------------------------------------- Matrix
+template <typename T>
+Matrix<T>::Matrix(const CSR<T>& A, std::shared_ptr<L2GMap> col_map,
+                  std::shared_ptr<L2GMap> row_map)
+    : _rows(A.rows), _cols(A.cols), _nnz(A.nonZeros()), _col_map(std::move(col_map)),
+      _row_map(std::move(row_map))
+{
+  if (static_cast<std::int64_t>(A.outer.size()) != A.rows + 1
+      || static_cast<std::int64_t>(A.inner.size()) != _nnz
+      || static_cast<std::int64_t>(A.values.size()) != _nnz)
+    throw std::runtime_error("spmv::Matrix: inconsistent CSR arrays");
+  if (_col_map && _col_map->local_size() + _col_map->num_ghosts() != A.cols)
+    throw std::runtime_error("spmv::Matrix: column map size differs from the column count");
+  std::vector<double> v(A.values.begin(), A.values.end());
+  check(spmvx_matrix_create(dtype_of<T>(), A.rows, A.cols, _nnz, A.outer.data(), A.inner.data(),
+                            v.data(), &_h));
+}
+
+template <typename T>
+Matrix<T>::~Matrix()
+{
+  if (_h)
+    spmvx_matrix_destroy(_h);
+}
+
+template <typename T>
+Matrix<T>::Matrix(Matrix&& o) noexcept
+    : _h(o._h), _rows(o._rows), _cols(o._cols), _nnz(o._nnz), _col_map(std::move(o._col_map)),
+      _row_map(std::move(o._row_map))
+{
+  o._h = nullptr;
+}
+
+template <typename T>
+Vector<T> Matrix<T>::operator*(const Vector<T>& b) const
+{
+  if (static_cast<std::int64_t>(b.size()) != _cols)
+    throw std::runtime_error("spmv::Matrix::operator*: vector size differs from cols()");
+  Vector<T> y(static_cast<std::size_t>(_rows));
+  check(spmvx_spmv(_h, b.data(), y.data(), nullptr));
+  return y;
+}
+
+template <typename T>
+Vector<T> Matrix<T>::transpmult(const Vector<T>& b) const
+{
+  if (static_cast<std::int64_t>(b.size()) != _rows)
+    throw std::runtime_error("spmv::Matrix::transpmult: vector size differs from rows()");
+  Vector<T> y(static_cast<std::size_t>(_cols));
+  check(spmvx_spmv_t(_h, b.data(), y.data(), nullptr));
+  return y;
+}
+
+template class Matrix<double>;
+template class Matrix<float>;
+
+// ---------------------------------------------------------------- cg
+std::tuple<Vector<double>, int> cg(MPI_Comm comm, const Matrix<double>& A,
+                                   const Vector<double>& b, int max_its, double rtol)
+{
+  (void)comm_size(comm);
+  std::shared_ptr<const L2GMap> col = A.col_map(), row = A.row_map();
+  if (row && row->num_ghosts() > 0) // cg.cpp:21-23
+    throw std::runtime_error("spmv::cg - Error: A.row_map() has ghost entries");
+  const std::int64_t M = row ? row->local_size() : A.rows();
+  if (static_cast<std::int64_t>(b.size()) != M) // cg.cpp:27-28
+    throw std::runtime_error("spmv::cg - Error: b.rows() != A.rows()");
+  Vector<double> x(static_cast<std::size_t>(A.cols()));
+  std::int32_t its = 0;
+  check(spmvx_cg(A.handle(), col ? col->handle() : nullptr, b.data(), x.data(), max_its, rtol,
+                 &its, nullptr, nullptr));
+  return std::make_tuple(std::move(x), static_cast<int>(its));
+}
+
+} // namespace spmv

@@ -89,7 +89,9 @@ __device__ __forceinline__ VecN<NRHS> block_sum_n(VecN<NRHS> a, float (*red)[16]
 }
 
 // y = A p over rows [r0, r1) with G lanes per row; returns this thread's
-// share of p.y (non-zero only in group leaders).
+// share of p.y (non-zero only in group leaders).  Each lane loads up to U
+// (colidx, value) pairs, then issues their U gathers of p, before
+// accumulating: U independent memory requests in flight per lane.
 template <int NRHS, int G>
 __device__ __forceinline__ VecN<NRHS> batch_spmv(const int32_t* __restrict__ rowptr,
                                                  const int32_t* __restrict__ colidx,
@@ -97,20 +99,34 @@ __device__ __forceinline__ VecN<NRHS> batch_spmv(const int32_t* __restrict__ row
                                                  const float* __restrict__ p,
                                                  float* __restrict__ y, int32_t r0, int32_t r1)
 {
+  constexpr int U = 4;
   VecN<NRHS> pdy{};
   const int gl = threadIdx.x % G;
   const int rows_per_pass = blockDim.x / G;
   for (int32_t row = r0 + static_cast<int32_t>(threadIdx.x / G); row < r1; row += rows_per_pass)
   {
     VecN<NRHS> acc{};
-    const int32_t e1 = rowptr[row + 1];
-    for (int32_t e = rowptr[row] + gl; e < e1; e += G)
+    const int32_t e0 = rowptr[row], e1 = rowptr[row + 1];
+    for (int32_t e = e0 + gl; e < e1; e += U * G)
     {
-      const float a = val[e];
-      const VecN<NRHS> pc = ldn<NRHS>(p + static_cast<int64_t>(colidx[e]) * NRHS);
+      int32_t c[U];
+      float a[U];
 #pragma unroll
-      for (int j = 0; j < NRHS; ++j)
-        acc.v[j] += a * pc.v[j];
+      for (int u = 0; u < U; ++u)
+      {
+        const int32_t k = e + u * G;
+        c[u] = k < e1 ? colidx[k] : row;
+        a[u] = k < e1 ? val[k] : 0.0f;
+      }
+      VecN<NRHS> pc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        pc[u] = ldn<NRHS>(p + static_cast<int64_t>(c[u]) * NRHS);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int j = 0; j < NRHS; ++j)
+          acc.v[j] += a[u] * pc[u].v[j];
     }
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1)
@@ -129,33 +145,101 @@ __device__ __forceinline__ VecN<NRHS> batch_spmv(const int32_t* __restrict__ row
   return pdy;
 }
 
+// Bounded spin: a team barrier gives up (and flags the solve as failed)
+// after about this many polls (a few seconds) instead of waiting forever; a
+// legitimate wait lasts at most one team member's solve of another system.
+constexpr unsigned kSpinLimit = 1u << 22;
+
+// Barrier among the W workgroups of one team (all of them are running: a
+// workgroup joins a team only by taking one of its slots from the queue).
+// Arrivals are counted on a per-system counter; the t-th barrier of a solve
+// completes at W * t arrivals.  Agent-scope fences make the vector slices
+// and partial sums written before the barrier visible to every member.
+__device__ __forceinline__ void team_barrier(unsigned* bar, unsigned target, int* err)
+{
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    __threadfence();
+    atomicAdd(bar, 1u);
+    unsigned spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+    {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > kSpinLimit)
+      {
+        atomicExch(err, 1);
+        break;
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+// Team-wide sum: member m stores its block sum in slot m of buffer `which`
+// (two buffers alternate), then every member adds the W slots in member
+// order (identical result in all members).
+template <int NRHS>
+__device__ __forceinline__ VecN<NRHS> team_sum(VecN<NRHS> v, float* tred, int which, int W, int m,
+                                               unsigned* bar, unsigned& nbar, int* err,
+                                               float (*red)[16])
+{
+  v = block_sum_n<NRHS>(v, red);
+  if (W == 1)
+    return v;
+  float* buf = tred + static_cast<int64_t>(which) * W * NRHS;
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < NRHS; ++j)
+      buf[m * NRHS + j] = v.v[j];
+  team_barrier(bar, static_cast<unsigned>(W) * (++nbar), err);
+  VecN<NRHS> t{};
+  for (int q = 0; q < W; ++q)
+#pragma unroll
+    for (int j = 0; j < NRHS; ++j)
+      t.v[j] += buf[q * NRHS + j];
+  return t;
+}
+
 template <int NRHS>
 __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const float* __restrict__ val, const int32_t* __restrict__ sys_row,
-    const int32_t* __restrict__ order, const int8_t* __restrict__ gsel, int32_t nsys,
+    const int2* __restrict__ slots, int32_t nslots, const int32_t* __restrict__ team,
+    const int64_t* __restrict__ tred_off, const int8_t* __restrict__ gsel,
     const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
     float* __restrict__ p, float* __restrict__ y, int32_t max_its, float rtol2,
-    int32_t* __restrict__ its_out, unsigned* queue)
+    int32_t* __restrict__ its_out, unsigned* queue, unsigned* bars, float* tred_all, int* err)
 {
-  __shared__ int32_t job;
+  __shared__ int2 job;
   __shared__ float red[NRHS][16];
   for (;;)
   {
     __syncthreads();
     if (threadIdx.x == 0)
-      job = static_cast<int32_t>(atomicAdd(queue, 1u));
+    {
+      const unsigned q = atomicAdd(queue, 1u);
+      job = q < static_cast<unsigned>(nslots) ? slots[q] : make_int2(-1, 0);
+    }
     __syncthreads();
-    const int32_t jb = job;
-    if (jb >= nsys)
+    const int2 jb = job;
+    if (jb.x < 0)
       break;
-    const int32_t s = order[jb];
-    const int32_t r0 = sys_row[s], r1 = sys_row[s + 1];
+    const int32_t s = jb.x, m = jb.y, W = team[s];
+    const int32_t r0s = sys_row[s], r1s = sys_row[s + 1];
+    const int64_t ns = r1s - r0s;
+    // this member's rows
+    const int32_t lo = r0s + static_cast<int32_t>(ns * m / W);
+    const int32_t hi = r0s + static_cast<int32_t>(ns * (m + 1) / W);
     const int g = gsel[s];
+    unsigned* bar = bars + s;
+    float* tred = tred_all + tred_off[s];
+    unsigned nbar = 0;
 
     // x = 0, r = b, p = b, rr0 = ||b||^2 (cg.cpp:31-44)
     VecN<NRHS> rr{};
-    for (int32_t i = r0 + static_cast<int32_t>(threadIdx.x); i < r1; i += blockDim.x)
+    for (int32_t i = lo + static_cast<int32_t>(threadIdx.x); i < hi; i += blockDim.x)
     {
       const int64_t o = static_cast<int64_t>(i) * NRHS;
       const VecN<NRHS> bi = ldn<NRHS>(b + o);
@@ -166,7 +250,7 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
       for (int j = 0; j < NRHS; ++j)
         rr.v[j] += bi.v[j] * bi.v[j];
     }
-    rr = block_sum_n<NRHS>(rr, red);
+    rr = team_sum<NRHS>(rr, tred, 0, W, m, bar, nbar, err, red); // also publishes p
     const VecN<NRHS> rr0 = rr;
     bool active[NRHS];
     int32_t its[NRHS];
@@ -180,23 +264,23 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
     }
     for (int32_t k = 1; k <= max_its && any; ++k)
     {
-      __syncthreads(); // p complete
+      __syncthreads(); // p complete (within the workgroup; across the team: barrier below)
       VecN<NRHS> pdy;
       switch (g)
       {
-      case 1: pdy = batch_spmv<NRHS, 1>(rowptr, colidx, val, p, y, r0, r1); break;
-      case 2: pdy = batch_spmv<NRHS, 2>(rowptr, colidx, val, p, y, r0, r1); break;
-      case 4: pdy = batch_spmv<NRHS, 4>(rowptr, colidx, val, p, y, r0, r1); break;
-      case 8: pdy = batch_spmv<NRHS, 8>(rowptr, colidx, val, p, y, r0, r1); break;
-      default: pdy = batch_spmv<NRHS, 16>(rowptr, colidx, val, p, y, r0, r1); break;
+      case 1: pdy = batch_spmv<NRHS, 1>(rowptr, colidx, val, p, y, lo, hi); break;
+      case 2: pdy = batch_spmv<NRHS, 2>(rowptr, colidx, val, p, y, lo, hi); break;
+      case 4: pdy = batch_spmv<NRHS, 4>(rowptr, colidx, val, p, y, lo, hi); break;
+      case 8: pdy = batch_spmv<NRHS, 8>(rowptr, colidx, val, p, y, lo, hi); break;
+      default: pdy = batch_spmv<NRHS, 16>(rowptr, colidx, val, p, y, lo, hi); break;
       }
-      pdy = block_sum_n<NRHS>(pdy, red); // also orders y writes before the reads below
+      pdy = team_sum<NRHS>(pdy, tred, 1, W, m, bar, nbar, err, red);
       float alpha[NRHS];
 #pragma unroll
       for (int j = 0; j < NRHS; ++j)
         alpha[j] = active[j] ? rr.v[j] / pdy.v[j] : 0.0f;
       VecN<NRHS> rn{};
-      for (int32_t i = r0 + static_cast<int32_t>(threadIdx.x); i < r1; i += blockDim.x)
+      for (int32_t i = lo + static_cast<int32_t>(threadIdx.x); i < hi; i += blockDim.x)
       {
         const int64_t o = static_cast<int64_t>(i) * NRHS;
         VecN<NRHS> ri = ldn<NRHS>(r + o), xi = ldn<NRHS>(x + o);
@@ -211,7 +295,7 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
         stn<NRHS>(x + o, xi);
         stn<NRHS>(r + o, ri);
       }
-      rn = block_sum_n<NRHS>(rn, red);
+      rn = team_sum<NRHS>(rn, tred, 0, W, m, bar, nbar, err, red);
       float beta[NRHS];
       any = false;
 #pragma unroll
@@ -229,7 +313,7 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
       }
       if (!any)
         break;
-      for (int32_t i = r0 + static_cast<int32_t>(threadIdx.x); i < r1; i += blockDim.x)
+      for (int32_t i = lo + static_cast<int32_t>(threadIdx.x); i < hi; i += blockDim.x)
       {
         const int64_t o = static_cast<int64_t>(i) * NRHS;
         const VecN<NRHS> ri = ldn<NRHS>(r + o);
@@ -239,15 +323,29 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
           pi.v[j] = pi.v[j] * beta[j] + ri.v[j];
         stn<NRHS>(p + o, pi);
       }
+      if (W > 1)
+        team_barrier(bar, static_cast<unsigned>(W) * (++nbar), err); // p complete
     }
-    if (threadIdx.x == 0)
+    if (m == 0 && threadIdx.x == 0)
 #pragma unroll
       for (int j = 0; j < NRHS; ++j)
         its_out[static_cast<int64_t>(s) * NRHS + j] = its[j];
   }
 }
 
-__global__ void reset_counter(unsigned* c) { *c = 0u; }
+// queue = 0, every team's barrier counter = 0, err = 0 (a kernel, not a
+// memset node, so that graph replays are reliable)
+__global__ void batch_reset(unsigned* queue, unsigned* bars, int32_t nsys, int* err)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0)
+  {
+    *queue = 0u;
+    *err = 0;
+  }
+  for (int32_t k = i; k < nsys; k += gridDim.x * blockDim.x)
+    bars[k] = 0u;
+}
 
 int pow2_ceil(int v)
 {
@@ -267,16 +365,22 @@ struct spmvx_batch_s
   int32_t nsys = 0, nrhs = 0;
   int64_t nrows = 0, nnz = 0;
   std::vector<int32_t> h_sys_row, h_rowptr;
-  DevBuf rowptr, colidx, val, sys_row, order, gsel, scratch, its, queue;
+  DevBuf rowptr, colidx, val, sys_row, gsel, scratch, its, queue;
+  // team plan
+  DevBuf slots, team, tred_off, bars, tred, err;
+  int32_t nslots = 0;
+  bool plan_dirty = true;
+  // launch configuration (autotuned)
   int32_t block = 1024;
-  int32_t g_policy = 0; // 0 = per system from the mean row length, else fixed G
+  int32_t g_policy = 0;          // 0 = per system from the mean row length, else fixed G
+  int32_t rows_per_member = 16384; // team size = ceil(rows / rows_per_member), 0 = no teams
   int grid = 0;
   // captured graph and the arguments it was captured with
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   const void* cap_b = nullptr;
   void* cap_x = nullptr;
-  int32_t cap_its = -1, cap_block = -1, cap_policy = -1;
+  int32_t cap_its = -1;
   float cap_rtol2 = -1.0f;
   // the batch's own stream (graphs cannot be captured on the null stream),
   // ordered after / before the caller's stream with events
@@ -300,22 +404,75 @@ struct spmvx_batch_s
 namespace
 {
 
-void apply_policy(spmvx_batch_s* B)
+int batch_capacity(spmvx_batch_s* B)
 {
-  std::vector<int8_t> g(B->nsys);
-  for (int s = 0; s < B->nsys; ++s)
+  int per_cu = 0;
+  if (B->nrhs == 4)
+    SPMVX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_cg_kernel<4>, B->block, 0));
+  else if (B->nrhs == 2)
+    SPMVX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_cg_kernel<2>, B->block, 0));
+  else
+    SPMVX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_cg_kernel<1>, B->block, 0));
+  int dev = 0, ncu = 0;
+  SPMVX_HIP(hipGetDevice(&dev));
+  SPMVX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  return std::max(1, per_cu) * std::max(1, ncu);
+}
+
+// Lanes per row (per system) and the team plan: systems in decreasing size
+// order (LPT), each split over W workgroups (W <= capacity / 2 so that a team
+// can always be completed by running workgroups), one queue slot per member.
+void build_plan(spmvx_batch_s* B, int capacity)
+{
+  const int nsys = B->nsys;
+  std::vector<int8_t> g(nsys);
+  std::vector<int32_t> team(nsys);
+  std::vector<int64_t> work(nsys);
+  const int wmax = std::max(1, std::min(64, capacity / 2));
+  for (int s = 0; s < nsys; ++s)
   {
+    const int64_t rows = B->h_sys_row[s + 1] - B->h_sys_row[s];
+    const int64_t nz = B->h_rowptr[B->h_sys_row[s + 1]] - B->h_rowptr[B->h_sys_row[s]];
+    work[s] = rows + nz;
     if (B->g_policy > 0)
       g[s] = static_cast<int8_t>(B->g_policy);
     else
     {
-      const int64_t rows = B->h_sys_row[s + 1] - B->h_sys_row[s];
-      const int64_t nz = B->h_rowptr[B->h_sys_row[s + 1]] - B->h_rowptr[B->h_sys_row[s]];
       const int mean = rows > 0 ? static_cast<int>((nz + rows - 1) / rows) : 1;
       g[s] = static_cast<int8_t>(std::min(16, pow2_ceil(std::max(1, mean / 4))));
     }
+    int64_t w = 1;
+    if (B->rows_per_member > 0)
+      w = (rows + B->rows_per_member - 1) / B->rows_per_member;
+    team[s] = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>(w, wmax)));
   }
-  SPMVX_HIP(hipMemcpy(B->gsel.ptr, g.data(), g.size(), hipMemcpyHostToDevice));
+  std::vector<int32_t> order(nsys);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t c) { return work[a] / team[a] > work[c] / team[c]; });
+  std::vector<int2> slots;
+  for (int32_t s : order)
+    for (int32_t m = 0; m < team[s]; ++m)
+      slots.push_back(make_int2(s, m));
+  std::vector<int64_t> off(nsys + 1, 0);
+  for (int s = 0; s < nsys; ++s)
+    off[s + 1] = off[s] + 2LL * team[s] * B->nrhs;
+  B->nslots = static_cast<int32_t>(slots.size());
+  B->gsel.ensure(std::max(nsys, 1));
+  B->team.ensure(4 * std::max(nsys, 1));
+  B->tred_off.ensure(8 * std::max(nsys, 1));
+  B->slots.ensure(sizeof(int2) * std::max<size_t>(slots.size(), 1));
+  B->tred.ensure(4 * std::max<int64_t>(off[nsys], 1));
+  B->bars.ensure(4 * std::max(nsys, 1));
+  B->err.ensure(256);
+  if (nsys)
+  {
+    SPMVX_HIP(hipMemcpy(B->gsel.ptr, g.data(), nsys, hipMemcpyHostToDevice));
+    SPMVX_HIP(hipMemcpy(B->team.ptr, team.data(), 4 * nsys, hipMemcpyHostToDevice));
+    SPMVX_HIP(hipMemcpy(B->tred_off.ptr, off.data(), 8 * nsys, hipMemcpyHostToDevice));
+    SPMVX_HIP(hipMemcpy(B->slots.ptr, slots.data(), sizeof(int2) * slots.size(), hipMemcpyHostToDevice));
+  }
+  B->plan_dirty = false;
 }
 
 void launch_batch(spmvx_batch_s* B, const float* b, float* x, int32_t max_its, float rtol2,
@@ -325,12 +482,15 @@ void launch_batch(spmvx_batch_s* B, const float* b, float* x, int32_t max_its, f
   float* r = B->scratch.as<float>();
   float* p = r + vn;
   float* y = p + vn;
-  reset_counter<<<1, 1, 0, s>>>(B->queue.as<unsigned>());
+  batch_reset<<<std::max(1, std::min(64, (B->nsys + 255) / 256)), 256, 0, s>>>(
+      B->queue.as<unsigned>(), B->bars.as<unsigned>(), B->nsys, B->err.as<int>());
 #define SPMVX_BATCH(NR)                                                                        \
   batch_cg_kernel<NR><<<B->grid, B->block, 0, s>>>(                                           \
       B->rowptr.as<int32_t>(), B->colidx.as<int32_t>(), B->val.as<float>(),                    \
-      B->sys_row.as<int32_t>(), B->order.as<int32_t>(), B->gsel.as<int8_t>(), B->nsys, b, x,   \
-      r, p, y, max_its, rtol2, B->its.as<int32_t>(), B->queue.as<unsigned>())
+      B->sys_row.as<int32_t>(), B->slots.as<int2>(), B->nslots, B->team.as<int32_t>(),         \
+      B->tred_off.as<int64_t>(), B->gsel.as<int8_t>(), b, x, r, p, y, max_its, rtol2,          \
+      B->its.as<int32_t>(), B->queue.as<unsigned>(), B->bars.as<unsigned>(),                   \
+      B->tred.as<float>(), B->err.as<int>())
   if (B->nrhs == 4)
     SPMVX_BATCH(4);
   else if (B->nrhs == 2)
@@ -338,6 +498,16 @@ void launch_batch(spmvx_batch_s* B, const float* b, float* x, int32_t max_its, f
   else
     SPMVX_BATCH(1);
 #undef SPMVX_BATCH
+}
+
+void drop_graph(spmvx_batch_s* B)
+{
+  if (B->exec)
+    SPMVX_HIP(hipGraphExecDestroy(B->exec));
+  if (B->graph)
+    SPMVX_HIP(hipGraphDestroy(B->graph));
+  B->exec = nullptr;
+  B->graph = nullptr;
 }
 
 } // namespace
@@ -385,22 +555,9 @@ int spmvx_batch_create(int32_t nsys, const int32_t* sys_row, const int32_t* rowp
     }
     B->sys_row.ensure(4 * (nsys + 1));
     SPMVX_HIP(hipMemcpy(B->sys_row.ptr, sys_row, 4 * (nsys + 1), hipMemcpyHostToDevice));
-    // LPT order: largest (rows + entries) first
-    std::vector<int32_t> order(nsys);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
-      const int64_t wa = (sys_row[a + 1] - sys_row[a]) + (rowptr[sys_row[a + 1]] - rowptr[sys_row[a]]);
-      const int64_t wc = (sys_row[c + 1] - sys_row[c]) + (rowptr[sys_row[c + 1]] - rowptr[sys_row[c]]);
-      return wa > wc;
-    });
-    B->order.ensure(4 * std::max(nsys, 1));
-    if (nsys)
-      SPMVX_HIP(hipMemcpy(B->order.ptr, order.data(), 4 * nsys, hipMemcpyHostToDevice));
-    B->gsel.ensure(std::max(nsys, 1));
     B->scratch.ensure(sizeof(float) * 3 * static_cast<size_t>(std::max<int64_t>(nrows, 1)) * nrhs);
     B->its.ensure(4 * static_cast<size_t>(std::max(nsys, 1)) * nrhs);
     B->queue.ensure(256);
-    apply_policy(B.get());
     *out = B.release();
   });
 }
@@ -410,7 +567,8 @@ int spmvx_batch_destroy(spmvx_batch B)
   return guarded([&] { delete B; });
 }
 
-int spmvx_batch_set_config(spmvx_batch B, int32_t block, int32_t lanes_per_row)
+int spmvx_batch_set_config(spmvx_batch B, int32_t block, int32_t lanes_per_row,
+                           int32_t rows_per_member)
 {
   return guarded([&] {
     SPMVX_REQUIRE(B != nullptr, "NULL batch");
@@ -418,9 +576,15 @@ int spmvx_batch_set_config(spmvx_batch B, int32_t block, int32_t lanes_per_row)
     SPMVX_REQUIRE(lanes_per_row == 0 || lanes_per_row == 1 || lanes_per_row == 2
                       || lanes_per_row == 4 || lanes_per_row == 8 || lanes_per_row == 16,
                   "lanes_per_row must be 0 (per system), 1, 2, 4, 8 or 16");
-    B->block = block;
-    B->g_policy = lanes_per_row;
-    apply_policy(B);
+    SPMVX_REQUIRE(rows_per_member >= 0, "rows_per_member must be >= 0");
+    if (block != B->block || lanes_per_row != B->g_policy || rows_per_member != B->rows_per_member)
+    {
+      B->block = block;
+      B->g_policy = lanes_per_row;
+      B->rows_per_member = rows_per_member;
+      B->plan_dirty = true;
+      drop_graph(B);
+    }
   });
 }
 
@@ -434,18 +598,14 @@ int spmvx_batch_solve(spmvx_batch B, const float* b, float* x, int32_t max_its, 
     hipStream_t s = as_stream(stream);
     if (B->nsys == 0)
       return;
-    // persistent grid: every workgroup resident at once
-    int per_cu = 0;
-    if (B->nrhs == 4)
-      SPMVX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_cg_kernel<4>, B->block, 0));
-    else if (B->nrhs == 2)
-      SPMVX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_cg_kernel<2>, B->block, 0));
-    else
-      SPMVX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_cg_kernel<1>, B->block, 0));
-    int dev = 0, ncu = 0;
-    SPMVX_HIP(hipGetDevice(&dev));
-    SPMVX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    B->grid = std::max(1, std::min(B->nsys, std::max(1, per_cu) * ncu));
+    if (B->plan_dirty)
+    {
+      const int cap = batch_capacity(B);
+      build_plan(B, cap);
+      // persistent grid: every workgroup resident at once
+      B->grid = std::max(1, std::min(B->nslots, cap));
+      drop_graph(B);
+    }
     const float rtol2 = static_cast<float>(rtol * rtol);
     if (!B->own)
     {
@@ -453,30 +613,20 @@ int spmvx_batch_solve(spmvx_batch B, const float* b, float* x, int32_t max_its, 
       SPMVX_HIP(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming));
       SPMVX_HIP(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming));
     }
-    const bool fresh = !B->exec || B->cap_b != b || B->cap_x != x || B->cap_its != max_its
-                       || B->cap_rtol2 != rtol2 || B->cap_block != B->block
-                       || B->cap_policy != B->g_policy;
-    if (fresh)
+    if (!B->exec || B->cap_b != b || B->cap_x != x || B->cap_its != max_its
+        || B->cap_rtol2 != rtol2)
     {
-      if (B->exec)
-        SPMVX_HIP(hipGraphExecDestroy(B->exec));
-      if (B->graph)
-        SPMVX_HIP(hipGraphDestroy(B->graph));
-      B->exec = nullptr;
-      B->graph = nullptr;
+      drop_graph(B);
       SPMVX_HIP(hipStreamBeginCapture(B->own, hipStreamCaptureModeThreadLocal));
       launch_batch(B, b, x, max_its, rtol2, B->own);
       hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(B->own, &g);
-      SPMVX_HIP(e);
+      SPMVX_HIP(hipStreamEndCapture(B->own, &g));
       B->graph = g;
       SPMVX_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
       B->cap_b = b;
       B->cap_x = x;
       B->cap_its = max_its;
       B->cap_rtol2 = rtol2;
-      B->cap_block = B->block;
-      B->cap_policy = B->g_policy;
     }
     SPMVX_HIP(hipEventRecord(B->ev_in, s));
     SPMVX_HIP(hipStreamWaitEvent(B->own, B->ev_in, 0));
@@ -485,15 +635,20 @@ int spmvx_batch_solve(spmvx_batch B, const float* b, float* x, int32_t max_its, 
     SPMVX_HIP(hipStreamWaitEvent(s, B->ev_out, 0));
     if (its_host)
     {
+      int err = 0;
       SPMVX_HIP(hipMemcpyAsync(its_host, B->its.ptr, 4 * static_cast<size_t>(B->nsys) * B->nrhs,
                                hipMemcpyDeviceToHost, s));
+      SPMVX_HIP(hipMemcpyAsync(&err, B->err.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
       SPMVX_HIP(hipStreamSynchronize(s));
+      if (err)
+        throw Error(SPMVX_ERR_STATE, "batched CG: a team barrier timed out (results invalid)");
     }
   });
 }
 
 int spmvx_batch_info(spmvx_batch B, int32_t* nsys, int64_t* nrows, int64_t* nnz, int32_t* nrhs,
-                     int32_t* block, int32_t* lanes_per_row, int32_t* grid)
+                     int32_t* block, int32_t* lanes_per_row, int32_t* rows_per_member,
+                     int32_t* grid, int32_t* nslots)
 {
   return guarded([&] {
     SPMVX_REQUIRE(B != nullptr, "NULL batch");
@@ -503,7 +658,9 @@ int spmvx_batch_info(spmvx_batch B, int32_t* nsys, int64_t* nrows, int64_t* nnz,
     if (nrhs) *nrhs = B->nrhs;
     if (block) *block = B->block;
     if (lanes_per_row) *lanes_per_row = B->g_policy;
+    if (rows_per_member) *rows_per_member = B->rows_per_member;
     if (grid) *grid = B->grid;
+    if (nslots) *nslots = B->nslots;
   });
 }
 

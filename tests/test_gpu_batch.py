@@ -22,10 +22,11 @@ def rhs(nrows, nrhs, seed):
     return P.uniform(seed, nrows * nrhs, -1, 1).astype(np.float32).reshape(nrows, nrhs)
 
 
-@pytest.mark.parametrize("nrhs", [4, 2, 1])
-def test_batch_matches_oracle(nrhs):
+@pytest.mark.parametrize("nrhs,rpm", [(4, 0), (4, 700), (2, 1000), (1, 16384)])
+def test_batch_matches_oracle(nrhs, rpm):
     systems = make_systems([1500, 700, 3000, 2200, 999, 4100, 1800, 640], 100)
     B = S.Batch(systems, nrhs=nrhs)
+    B.set_config(1024, 0, rpm)
     b = rhs(B.nrows, nrhs, 7)
     x, its = B.solve(torch.from_numpy(b).to(DEV), max_its=2000, rtol=1e-6)
     xh = x.cpu().numpy()
@@ -62,13 +63,14 @@ def test_batch_every_config_and_replay():
     bd = torch.from_numpy(bh).to(DEV)
     ref, its0 = B.solve(bd, max_its=1000, rtol=1e-6)
     for bs in T.BATCH_BLOCKS:
-        for g in T.BATCH_LANES:
-            B.set_config(bs, g)
-            x, its = B.solve(bd, max_its=1000, rtol=1e-6)
-            assert np.max(np.abs(its - its0)) <= 2
-            assert np.max(true_rel_residuals(systems, B, bh, x.cpu().numpy())) <= 5e-6
+        for g in T.BATCH_LANES + (1, 16):
+            for rpm in (0, 300, 1000):  # 0: one workgroup per system; else teams
+                B.set_config(bs, g, rpm)
+                x, its = B.solve(bd, max_its=1000, rtol=1e-6)
+                assert np.max(np.abs(its - its0)) <= 2
+                assert np.max(true_rel_residuals(systems, B, bh, x.cpu().numpy())) <= 5e-6
     # graph replays: same answer every time, with eager work in between
-    B.set_config(256, 0)
+    B.set_config(256, 0, 500)
     out = torch.empty_like(bd)
     first, _ = B.solve(bd, out, max_its=1000, rtol=1e-6)
     first = first.clone()
@@ -96,8 +98,10 @@ def test_autotuner_caches(tmp_path, monkeypatch):
     B = S.Batch(systems, nrhs=4)
     bd = torch.from_numpy(rhs(B.nrows, 4, 2)).to(DEV)
     best, table = T.tune_batch(B, bd, max_its=500, rtol=1e-6, reps=1)
-    assert table is not None and len(table) == len(T.BATCH_BLOCKS) * len(T.BATCH_LANES)
+    assert table is not None
+    assert len(table) == len(T.BATCH_BLOCKS) * len(T.BATCH_LANES) * len(T.BATCH_ROWS_PER_MEMBER)
     cfg = B.config()
-    assert (cfg["block"], cfg["lanes_per_row"]) == (best["block"], best["lanes_per_row"])
+    assert (cfg["block"], cfg["lanes_per_row"], cfg["rows_per_member"]) == \
+        (best["block"], best["lanes_per_row"], best["rows_per_member"])
     best2, table2 = T.tune_batch(B, bd, max_its=500, rtol=1e-6, reps=1)
     assert table2 is None and best2 == best

@@ -1,0 +1,52 @@
+"""The C++ drop-in API (spmv::Matrix, spmv::L2GMap, spmv::cg) through the
+demo drivers built from spmv_amd/demos/, checked against the oracle."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from spmv_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "spmv_amd", "bin")
+
+
+def run(exe, *args):
+    path = os.path.join(BIN, exe)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} was not built")
+    out = subprocess.run([path, *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def field(text, name):
+    return float(re.search(rf"^{re.escape(name)} = (\S+)$", text, re.M).group(1))
+
+
+def test_matvecmain_matches_oracle():
+    N, n_apply = 20000, 50
+    out = run("matvecmain", N, n_apply)
+    A = O.gen_demo1d(N)
+    z = np.arange(N, dtype=np.float64) / N
+    v = np.exp(-10 * np.power(5 * (z - 0.5), 2.0))
+    for _ in range(n_apply):
+        v = O.spmv(A, v)
+    want = float(np.sum(v * v))
+    # the SpMVs are bit-identical; the initial exp/pow may differ by an ulp
+    assert abs(field(out, "norm") - want) <= 1e-12 * want
+
+
+def test_cgmain_matches_oracle():
+    n = 128
+    out = run("cgmain", n, 1e-8, 10000, 1)
+    A = O.gen_stencil(5, n, n)
+    xt = O.uniform(1, n * n, -1, 1)
+    b = O.spmv(A, xt)
+    x, k, _ = O.cg(A, b, 10000, 1e-8)
+    assert abs(field(out, "iterations") - k) <= 1
+    assert field(out, "rel.residual") <= 1e-7
+    assert abs(field(out, "x.norm") - np.linalg.norm(x)) <= 1e-8 * np.linalg.norm(x)
