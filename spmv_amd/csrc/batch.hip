@@ -202,25 +202,51 @@ __device__ __forceinline__ VecN<NRHS> team_sum(VecN<NRHS> v, float* tred, int wh
   return t;
 }
 
+// Index of the XCD (accelerator complex die) the calling wave runs on, from
+// the XCC_ID hardware register.  Used only to pick a work queue: a wrong
+// value costs locality, never correctness.
+__device__ __forceinline__ int current_xcd()
+{
+  // s_getreg(HW_REG_XCC_ID = 20, offset 0, 4 bits)
+  return static_cast<int>(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20)) & 0xF;
+}
+
 template <int NRHS>
 __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const float* __restrict__ val, const int32_t* __restrict__ sys_row,
-    const int2* __restrict__ slots, int32_t nslots, const int32_t* __restrict__ team,
-    const int64_t* __restrict__ tred_off, const int8_t* __restrict__ gsel,
-    const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
-    float* __restrict__ p, float* __restrict__ y, int32_t max_its, float rtol2,
-    int32_t* __restrict__ its_out, unsigned* queue, unsigned* bars, float* tred_all, int* err)
+    const int2* __restrict__ slots, const int32_t* __restrict__ qoff, int32_t nq,
+    const int32_t* __restrict__ team, const int64_t* __restrict__ tred_off,
+    const int8_t* __restrict__ gsel, const float* __restrict__ b, float* __restrict__ x,
+    float* __restrict__ r, float* __restrict__ p, float* __restrict__ y, int32_t max_its,
+    float rtol2, int32_t* __restrict__ its_out, unsigned* queue, unsigned* bars,
+    float* tred_all, int* err)
 {
   __shared__ int2 job;
   __shared__ float red[NRHS][16];
+  // queues are per XCD: members of one team share an L2; a workgroup whose
+  // own queue is empty takes slots from the next ones
+  const int home = current_xcd() % nq;
+  int t = 0;
   for (;;)
   {
     __syncthreads();
     if (threadIdx.x == 0)
     {
-      const unsigned q = atomicAdd(queue, 1u);
-      job = q < static_cast<unsigned>(nslots) ? slots[q] : make_int2(-1, 0);
+      int2 jbv = make_int2(-1, 0);
+      while (t < nq)
+      {
+        const int q = (home + t) % nq;
+        const unsigned n = static_cast<unsigned>(qoff[q + 1] - qoff[q]);
+        const unsigned i = atomicAdd(&queue[q], 1u);
+        if (i < n)
+        {
+          jbv = slots[qoff[q] + i];
+          break;
+        }
+        ++t;
+      }
+      job = jbv;
     }
     __syncthreads();
     const int2 jb = job;
@@ -333,16 +359,15 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
   }
 }
 
-// queue = 0, every team's barrier counter = 0, err = 0 (a kernel, not a
+// queues = 0, every team's barrier counter = 0, err = 0 (a kernel, not a
 // memset node, so that graph replays are reliable)
-__global__ void batch_reset(unsigned* queue, unsigned* bars, int32_t nsys, int* err)
+__global__ void batch_reset(unsigned* queue, int32_t nq, unsigned* bars, int32_t nsys, int* err)
 {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nq)
+    queue[i] = 0u;
   if (i == 0)
-  {
-    *queue = 0u;
     *err = 0;
-  }
   for (int32_t k = i; k < nsys; k += gridDim.x * blockDim.x)
     bars[k] = 0u;
 }
@@ -367,8 +392,9 @@ struct spmvx_batch_s
   std::vector<int32_t> h_sys_row, h_rowptr;
   DevBuf rowptr, colidx, val, sys_row, gsel, scratch, its, queue;
   // team plan
-  DevBuf slots, team, tred_off, bars, tred, err;
+  DevBuf slots, qoff, team, tred_off, bars, tred, err;
   int32_t nslots = 0;
+  int32_t nq = kNumXcd; // work queues, one per XCD
   bool plan_dirty = true;
   // launch configuration (autotuned)
   int32_t block = 1024;
@@ -428,7 +454,10 @@ void build_plan(spmvx_batch_s* B, int capacity)
   std::vector<int8_t> g(nsys);
   std::vector<int32_t> team(nsys);
   std::vector<int64_t> work(nsys);
-  const int wmax = std::max(1, std::min(64, capacity / 2));
+  // every XCD queue may hold one team still gathering members; capping teams
+  // at capacity / (2 * queues) leaves enough running workgroups to complete
+  // all of them
+  const int wmax = std::max(1, std::min(64, capacity / (2 * B->nq)));
   for (int s = 0; s < nsys; ++s)
   {
     const int64_t rows = B->h_sys_row[s + 1] - B->h_sys_row[s];
@@ -446,14 +475,28 @@ void build_plan(spmvx_batch_s* B, int capacity)
       w = (rows + B->rows_per_member - 1) / B->rows_per_member;
     team[s] = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>(w, wmax)));
   }
+  // LPT: systems by decreasing work go to the XCD queue with the least work
   std::vector<int32_t> order(nsys);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
-                   [&](int32_t a, int32_t c) { return work[a] / team[a] > work[c] / team[c]; });
-  std::vector<int2> slots;
+                   [&](int32_t a, int32_t c) { return work[a] > work[c]; });
+  const int nq = B->nq;
+  std::vector<std::vector<int2>> qs(nq);
+  std::vector<int64_t> load(nq, 0);
   for (int32_t s : order)
+  {
+    const int q = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+    load[q] += work[s];
     for (int32_t m = 0; m < team[s]; ++m)
-      slots.push_back(make_int2(s, m));
+      qs[q].push_back(make_int2(s, m));
+  }
+  std::vector<int2> slots;
+  std::vector<int32_t> qoff(nq + 1, 0);
+  for (int q = 0; q < nq; ++q)
+  {
+    slots.insert(slots.end(), qs[q].begin(), qs[q].end());
+    qoff[q + 1] = static_cast<int32_t>(slots.size());
+  }
   std::vector<int64_t> off(nsys + 1, 0);
   for (int s = 0; s < nsys; ++s)
     off[s + 1] = off[s] + 2LL * team[s] * B->nrhs;
@@ -462,6 +505,8 @@ void build_plan(spmvx_batch_s* B, int capacity)
   B->team.ensure(4 * std::max(nsys, 1));
   B->tred_off.ensure(8 * std::max(nsys, 1));
   B->slots.ensure(sizeof(int2) * std::max<size_t>(slots.size(), 1));
+  B->qoff.ensure(4 * (nq + 1));
+  SPMVX_HIP(hipMemcpy(B->qoff.ptr, qoff.data(), 4 * (nq + 1), hipMemcpyHostToDevice));
   B->tred.ensure(4 * std::max<int64_t>(off[nsys], 1));
   B->bars.ensure(4 * std::max(nsys, 1));
   B->err.ensure(256);
@@ -483,11 +528,12 @@ void launch_batch(spmvx_batch_s* B, const float* b, float* x, int32_t max_its, f
   float* p = r + vn;
   float* y = p + vn;
   batch_reset<<<std::max(1, std::min(64, (B->nsys + 255) / 256)), 256, 0, s>>>(
-      B->queue.as<unsigned>(), B->bars.as<unsigned>(), B->nsys, B->err.as<int>());
+      B->queue.as<unsigned>(), B->nq, B->bars.as<unsigned>(), B->nsys, B->err.as<int>());
 #define SPMVX_BATCH(NR)                                                                        \
   batch_cg_kernel<NR><<<B->grid, B->block, 0, s>>>(                                           \
       B->rowptr.as<int32_t>(), B->colidx.as<int32_t>(), B->val.as<float>(),                    \
-      B->sys_row.as<int32_t>(), B->slots.as<int2>(), B->nslots, B->team.as<int32_t>(),         \
+      B->sys_row.as<int32_t>(), B->slots.as<int2>(), B->qoff.as<int32_t>(), B->nq,            \
+      B->team.as<int32_t>(),                                                                   \
       B->tred_off.as<int64_t>(), B->gsel.as<int8_t>(), b, x, r, p, y, max_its, rtol2,          \
       B->its.as<int32_t>(), B->queue.as<unsigned>(), B->bars.as<unsigned>(),                   \
       B->tred.as<float>(), B->err.as<int>())

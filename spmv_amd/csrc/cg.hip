@@ -5,9 +5,9 @@
 //   csr_stream<kDot>   y = A p, p.y, alpha = rnorm_old / p.y     cg.cpp:55-62
 //   cg_update_r        r -= alpha y, ||r||^2, beta, k, converged cg.cpp:66-73,78
 //   cg_update_xp       x += alpha p, p = beta p + r, stop flag   cg.cpp:65,76
-// Every kernel returns immediately once the device-side `active` flag is
-// cleared, so the host enqueues iterations in chunks and polls the flag
-// between chunks; the iteration count is exact.
+// Every kernel returns immediately once the device-side `stopped` flag is
+// set, so the host replays a captured graph of a fixed number of iterations
+// and polls the flag between replays; the iteration count is exact.
 #include <cmath>
 #include <cstring>
 
@@ -67,7 +67,13 @@ __global__ void cg_state_init(CgState<Acc>* st, int32_t kmax, Acc rtol2)
   *st = s;
 }
 
+constexpr int kGraphChunk = 16; // iterations per captured graph
+
 // Host driver of one CG solve.  x has ncols entries; b has nrows.
+// A chunk of kGraphChunk iterations is captured once (per matrix, x, history
+// buffer and ghost map) and replayed; the kernels are gated by the device
+// state, so replays past convergence are no-ops.  Two replays are kept in
+// flight while the host polls the stop flag of the older one.
 template <typename V, typename X, typename Acc>
 int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t vec_dtype,
            const X* b, X* x, int32_t kmax, Acc rtol, Acc* hist_dev, hipStream_t s)
@@ -85,40 +91,94 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
   Acc* part_1 = static_cast<Acc*>(w.red_vec1);
   const int vg = vec_grid(n);
   const int g0 = vec_grid(ncols > n ? ncols : n);
-  cg_state_init<Acc><<<1, 1, 0, s>>>(st, kmax, rtol * rtol);
-  cg_init<X, Acc><<<g0, kVecBS, 0, s>>>(n, ncols, b, x, r, p, part_0);
-  cg_init_finalize<Acc><<<1, kFinBS, 0, s>>>(part_0, g0, st, hist_dev);
-  check_launch("cg_init");
-  auto* flag = static_cast<CgState<Acc>*>(A->host_flag.ensure(sizeof(CgState<Acc>)));
-  int32_t it = 0;
-  int32_t chunk = 4;
-  while (it < kmax)
+  if (!w.stream)
   {
-    const int32_t last = (kmax - it) < chunk ? kmax : it + chunk;
-    for (++it; it <= last; ++it)
-    {
-      if (col_map)
-        l2gmap_update_async(col_map, vec_dtype, p, s); // cg.cpp:55
-      const int64_t nparts = launch_csr_stream<V, X, Acc, kDot>(
-          A->variant, rb.nblocks, s, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
-          static_cast<const V*>(A->values.ptr), p, y, rb.desc.as<int2>(), nullptr, part_s, st, it);
-      cg_alpha_finalize<Acc><<<1, kFinBS, 0, s>>>(part_s, nparts, st, it);
-      cg_update_r<X, Acc><<<vg, kVecBS, 0, s>>>(n, r, y, part_1, st, it);
-      cg_beta_finalize<Acc><<<1, kFinBS, 0, s>>>(part_1, vg, st, hist_dev, it);
-      cg_update_xp<X, Acc><<<vg, kVecBS, 0, s>>>(n, x, p, r, st, it);
-    }
-    --it;
-    check_launch("cg iteration");
-    SPMVX_HIP(hipMemcpyAsync(flag, st, sizeof(CgState<Acc>), hipMemcpyDeviceToHost, s));
-    SPMVX_HIP(hipStreamSynchronize(s));
-    if (flag->stopped)
-      break;
-    if (chunk < 32)
-      chunk *= 2;
+    SPMVX_HIP(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+    SPMVX_HIP(hipEventCreateWithFlags(&w.ev_in, hipEventDisableTiming));
+    SPMVX_HIP(hipEventCreateWithFlags(&w.ev_poll[0], hipEventDisableTiming));
+    SPMVX_HIP(hipEventCreateWithFlags(&w.ev_poll[1], hipEventDisableTiming));
+    w.poll.ensure(2 * sizeof(int32_t));
   }
-  SPMVX_HIP(hipMemcpyAsync(flag, st, sizeof(CgState<Acc>), hipMemcpyDeviceToHost, s));
-  SPMVX_HIP(hipStreamSynchronize(s));
-  return flag->k;
+  hipStream_t q = w.stream;
+  if (col_map)
+    l2gmap_prepare_device(col_map);
+
+  // one CG iteration (cg.cpp:52-80); every kernel is gated by the state
+  auto iteration = [&](hipStream_t qq) {
+    if (col_map)
+      l2gmap_update_async(col_map, vec_dtype, p, qq); // cg.cpp:55
+    const int64_t nparts = launch_csr_stream<V, X, Acc, kDot>(
+        A->variant, rb.nblocks, qq, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
+        static_cast<const V*>(A->values.ptr), p, y, rb.desc.as<int2>(), nullptr, part_s, st, 1);
+    cg_alpha_finalize<Acc><<<1, kFinBS, 0, qq>>>(part_s, nparts, st, 1);
+    cg_update_r<X, Acc><<<vg, kVecBS, 0, qq>>>(n, r, y, part_1, st, 1);
+    cg_beta_finalize<Acc><<<1, kFinBS, 0, qq>>>(part_1, vg, st, hist_dev, 1);
+    cg_update_xp<X, Acc><<<vg, kVecBS, 0, qq>>>(n, x, p, r, st, 1);
+  };
+
+  const void* halo = col_map ? col_map->h_halo.ptr : nullptr;
+  const int32_t halo_dt = col_map ? col_map->halo_dtype : -2;
+  if (!w.exec || w.g_x != x || w.g_hist != hist_dev || w.g_map != col_map || w.g_halo != halo
+      || w.g_halo_dtype != halo_dt || w.g_variant != A->variant || w.g_mem != w.mem.ptr
+      || w.g_values != A->values.ptr)
+  {
+    if (w.exec)
+      SPMVX_HIP(hipGraphExecDestroy(w.exec));
+    if (w.graph)
+      SPMVX_HIP(hipGraphDestroy(w.graph));
+    w.exec = nullptr;
+    w.graph = nullptr;
+    SPMVX_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < kGraphChunk; ++j)
+      iteration(q);
+    hipGraph_t g = nullptr;
+    const hipError_t cap = hipStreamEndCapture(q, &g);
+    check_launch("cg iteration capture");
+    SPMVX_HIP(cap);
+    w.graph = g;
+    SPMVX_HIP(hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0));
+    w.g_x = x;
+    w.g_hist = hist_dev;
+    w.g_map = col_map;
+    w.g_halo = halo;
+    w.g_halo_dtype = halo_dt;
+    w.g_variant = A->variant;
+    w.g_mem = w.mem.ptr;
+    w.g_values = A->values.ptr;
+  }
+
+  // the caller's stream first (b and x may have just been written on it)
+  SPMVX_HIP(hipEventRecord(w.ev_in, s));
+  SPMVX_HIP(hipStreamWaitEvent(q, w.ev_in, 0));
+  cg_state_init<Acc><<<1, 1, 0, q>>>(st, kmax, rtol * rtol);
+  cg_init<X, Acc><<<g0, kVecBS, 0, q>>>(n, ncols, b, x, r, p, part_0);
+  cg_init_finalize<Acc><<<1, kFinBS, 0, q>>>(part_0, g0, st, hist_dev);
+  check_launch("cg_init");
+  auto* poll = static_cast<int32_t*>(w.poll.ptr);
+  int32_t launched = 0;
+  int slot = 0, inflight = 0;
+  bool stopped = kmax <= 0;
+  while (!stopped && launched < kmax)
+  {
+    SPMVX_HIP(hipGraphLaunch(w.exec, q));
+    launched += kGraphChunk;
+    SPMVX_HIP(hipMemcpyAsync(&poll[slot], &st->stopped, sizeof(int32_t), hipMemcpyDeviceToHost, q));
+    SPMVX_HIP(hipEventRecord(w.ev_poll[slot], q));
+    slot ^= 1;
+    if (++inflight == 2)
+    {
+      SPMVX_HIP(hipEventSynchronize(w.ev_poll[slot])); // the older replay
+      stopped = poll[slot] != 0;
+      inflight = 1;
+    }
+  }
+  CgState<Acc> fin{};
+  SPMVX_HIP(hipMemcpyAsync(A->host_flag.ensure(sizeof(CgState<Acc>)), st, sizeof(CgState<Acc>),
+                           hipMemcpyDeviceToHost, q));
+  SPMVX_HIP(hipStreamSynchronize(q));
+  std::memcpy(&fin, A->host_flag.ptr, sizeof(fin));
+  (void)s;
+  return fin.k;
 }
 
 // ---------------------------------------------------------------- refinement

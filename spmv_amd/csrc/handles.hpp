@@ -113,9 +113,40 @@ inline int vec_grid(int64_t n)
   return static_cast<int>(g);
 }
 
-// Per-precision scratch of the CG solvers, owned by a matrix handle.
+// Per-precision scratch of the CG solvers, owned by a matrix handle,
+// including the captured graph of a chunk of iterations and the stream it
+// runs on (graphs cannot be captured on the null stream).
 struct CgWorkspace
 {
+  CgWorkspace() = default;
+  CgWorkspace(const CgWorkspace&) = delete;
+  CgWorkspace& operator=(const CgWorkspace&) = delete;
+  ~CgWorkspace()
+  {
+    if (exec)
+      (void)hipGraphExecDestroy(exec);
+    if (graph)
+      (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : {ev_in, ev_poll[0], ev_poll[1]})
+      if (e)
+        (void)hipEventDestroy(e);
+    if (stream)
+      (void)hipStreamDestroy(stream);
+  }
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr;
+  hipEvent_t ev_poll[2] = {nullptr, nullptr};
+  PinnedBuf poll;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* g_x = nullptr;   // arguments the graph was captured with
+  const void* g_hist = nullptr;
+  const void* g_map = nullptr;
+  const void* g_halo = nullptr;
+  int32_t g_halo_dtype = -2;
+  int32_t g_variant = -1;
+  const void* g_mem = nullptr;
+  const void* g_values = nullptr;
   DevBuf mem;
   int64_t n = 0, ncols = 0, nblocks = 0;
   size_t elem = 0;

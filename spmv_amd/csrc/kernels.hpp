@@ -33,13 +33,13 @@ struct bf16_t
 
 
 // ---------------------------------------------------------------- CG state
-// Scalars of one CG solve, resident on the device.  The host enqueues the
-// kernels of iteration `it` (1-based) with `it` as an argument; they run only
-// while the solve has completed it-1 iterations and has not stopped, so the
-// iteration count is exact however many iterations are enqueued.  No kernel
-// synchronises with another inside a launch: reductions write per-block
-// partial sums and a one-workgroup finalize kernel (after the kernel
-// boundary) combines them in a fixed order.
+// Scalars of one CG solve, resident on the device.  Every kernel of an
+// iteration checks the state at its start and returns at once when the solve
+// has stopped, so the host (or a replayed graph of a fixed number of
+// iterations) can enqueue more iterations than are needed and the iteration
+// count is still exact.  No kernel synchronises with another inside a launch:
+// reductions write per-block partial sums and a one-workgroup finalize
+// kernel (after the kernel boundary) combines them in a fixed order.
 template <typename Acc>
 struct CgState
 {
@@ -48,18 +48,20 @@ struct CgState
   Acc alpha;
   Acc beta;
   Acc rtol2;
-  Acc last;    // last reduction result (residual kernels)
-  int32_t k;       // iterations completed
+  Acc last;           // last reduction result (residual kernels)
+  int32_t k;          // iterations completed
   int32_t kmax;
-  int32_t stopped; // converged or k == kmax
+  int32_t stopped;    // converged or k == kmax
   int32_t converged;
+  int32_t xp_pending; // the residual update of iteration k ran; x/p update due
 };
 
-// Kernels of iteration `it` before the residual update run iff this holds.
+// Kernels before the residual update run iff this holds (live = 0 means
+// "not part of a CG iteration": always run).
 template <typename Acc>
-__device__ __forceinline__ bool cg_live(const CgState<Acc>* st, int32_t it)
+__device__ __forceinline__ bool cg_live(const CgState<Acc>* st, int32_t live)
 {
-  return it == 0 || (!st->stopped && st->k == it - 1);
+  return live == 0 || !st->stopped;
 }
 
 enum SpmvMode
@@ -157,6 +159,26 @@ __device__ __forceinline__ auto widen(V v)
     return v;
 }
 
+// Sequential sum prod[a] + prod[a+1] + ... + prod[e-1] starting from 0 (the
+// reference's per-row order), with the LDS reads issued four at a time.
+template <typename Acc>
+__device__ __forceinline__ Acc row_sum(const Acc* prod, int a, int e)
+{
+  Acc s = Acc(0);
+  int q = a;
+  for (; q + 4 <= e; q += 4)
+  {
+    const Acc p0 = prod[q], p1 = prod[q + 1], p2 = prod[q + 2], p3 = prod[q + 3];
+    s += p0;
+    s += p1;
+    s += p2;
+    s += p3;
+  }
+  for (; q < e; ++q)
+    s += prod[q];
+  return s;
+}
+
 // Kernel configuration: products staged per block (CHUNK), consecutive
 // entries each thread loads with one vector load per array (VEC; the entry
 // range of a block is widened down to a multiple of VEC, the arrays are
@@ -197,18 +219,27 @@ __global__ __launch_bounds__(kBS) void csr_stream(
     constexpr int STEP = kBS * VEC;
     for (int32_t g0 = kb; g0 < k1; g0 += STEP * UNROLL)
     {
+      // Branch-free loads and gathers: a vector past the block is replaced by
+      // the block's first vector, and entries outside the block (neighbouring
+      // blocks' entries or the zeroed padding) are valid columns, so every
+      // load can be issued before any result is needed; only the LDS store
+      // is predicated.
       int32_t c[UNROLL][VEC];
       V v[UNROLL][VEC];
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u)
       {
         const int32_t g = g0 + u * STEP + tid * VEC;
-        if (g < k1)
-        {
-          load_vec<NT>(colidx + g, c[u]);
-          load_vec<NT>(val + g, v[u]);
-        }
+        const int32_t gs = g < k1 ? g : kb;
+        load_vec<NT>(colidx + gs, c[u]);
+        load_vec<NT>(val + gs, v[u]);
       }
+      X xv[UNROLL][VEC];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u)
+#pragma unroll
+        for (int j = 0; j < VEC; ++j)
+          xv[u][j] = x[c[u][j]];
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u)
       {
@@ -217,17 +248,15 @@ __global__ __launch_bounds__(kBS) void csr_stream(
         for (int j = 0; j < VEC; ++j)
         {
           const int32_t k = g + j;
-          if (k >= k0 && k < k1)
-            prod[k - k0] = static_cast<Acc>(widen(v[u][j])) * static_cast<Acc>(x[c[u][j]]);
+          if (g < k1 && k >= k0 && k < k1)
+            prod[k - k0] = static_cast<Acc>(widen(v[u][j])) * static_cast<Acc>(xv[u][j]);
         }
       }
     }
     __syncthreads();
     if (row < r1)
     {
-      Acc s = Acc(0);
-      for (int q = ra - k0; q < re - k0; ++q)
-        s += prod[q];
+      const Acc s = row_sum(prod, ra - k0, re - k0);
       if (MODE == kResid)
       {
         const X yi = b[row] - static_cast<X>(s);
@@ -324,11 +353,9 @@ __device__ __forceinline__ void pipe_issue(PipeStage<V, VEC, PER>& S,
   for (int u = 0; u < PER; ++u)
   {
     const int32_t g = kb + (u * kBS + tid) * VEC;
-    if (g < d1.y)
-    {
-      load_vec<NT>(colidx + g, S.c[u]);
-      load_vec<NT>(val + g, S.v[u]);
-    }
+    const int32_t gs = g < d1.y ? g : kb; // stay inside the arrays, no branch
+    load_vec<NT>(colidx + gs, S.c[u]);
+    load_vec<NT>(val + gs, S.v[u]);
   }
 }
 
@@ -350,10 +377,7 @@ __device__ __forceinline__ void pipe_gather(const PipeStage<V, VEC, PER>& S,
     const int32_t g = kb + (u * kBS + static_cast<int32_t>(threadIdx.x)) * VEC;
 #pragma unroll
     for (int j = 0; j < VEC; ++j)
-    {
-      const int32_t k = g + j;
-      xg[u][j] = (k >= k0 && k < k1) ? x[S.c[u][j]] : X(0);
-    }
+      xg[u][j] = x[S.c[u][j]]; // entries outside the block hold valid columns
   }
 }
 
@@ -380,16 +404,14 @@ __device__ __forceinline__ void pipe_compute(const PipeStage<V, VEC, PER>& S,
       for (int j = 0; j < VEC; ++j)
       {
         const int32_t k = g + j;
-        if (k >= k0 && k < k1)
+        if (g < k1 && k >= k0 && k < k1)
           prod[k - k0] = static_cast<Acc>(widen(S.v[u][j])) * static_cast<Acc>(xg[u][j]);
       }
     }
     __syncthreads();
     if (row < r1)
     {
-      Acc s = Acc(0);
-      for (int q = S.ra - k0; q < S.re - k0; ++q)
-        s += prod[q];
+      const Acc s = row_sum(prod, S.ra - k0, S.re - k0);
       if (MODE == kResid)
       {
         const X yi = b[row] - static_cast<X>(s);
@@ -620,19 +642,25 @@ __global__ __launch_bounds__(kFinBS) void cg_init_finalize(const Acc* __restrict
     st->rnorm_old = s;
     st->k = 0;
     st->converged = 0;
+    st->xp_pending = 0;
     st->stopped = st->kmax > 0 ? 0 : 1;
     if (hist)
       hist[0] = s;
   }
 }
 
-// alpha = rnorm_old / (p.y)  (cg.cpp:59-62)
+// alpha = rnorm_old / (p.y)  (cg.cpp:59-62).  Also retires the previous
+// iteration's x/p update (which ran before this kernel in stream order).
 template <typename Acc>
 __global__ __launch_bounds__(kFinBS) void cg_alpha_finalize(const Acc* __restrict__ partials,
                                                             int64_t n, CgState<Acc>* st,
                                                             int32_t it)
 {
-  if (!cg_live(st, it))
+  const bool live = cg_live(st, it);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    st->xp_pending = 0;
+  if (!live)
     return;
   __shared__ Acc red[16];
   const Acc s = finalize_sum(partials, n, red);
@@ -687,17 +715,19 @@ __global__ __launch_bounds__(kFinBS) void cg_beta_finalize(const Acc* __restrict
   const Acc s = finalize_sum(partials, n, red);
   if (threadIdx.x == 0)
   {
+    const int32_t k = st->k + 1;
     st->beta = s / st->rnorm_old;
     st->rnorm_old = s;
-    st->k = it;
+    st->k = k;
     if (hist)
-      hist[it] = s;
+      hist[k] = s;
     st->converged = (s / st->rnorm0 < st->rtol2) ? 1 : 0;
-    st->stopped = (st->converged || it >= st->kmax) ? 1 : 0;
+    st->stopped = (st->converged || k >= st->kmax) ? 1 : 0;
+    st->xp_pending = 1;
   }
 }
 
-// x += alpha p; p = beta p + r (cg.cpp:65,76): runs iff iteration `it`
+// x += alpha p; p = beta p + r (cg.cpp:65,76): runs iff this iteration
 // completed its residual update (also on the iteration that stops).
 template <typename X, typename Acc>
 __global__ __launch_bounds__(kVecBS) void cg_update_xp(int64_t n, X* __restrict__ x,
@@ -705,7 +735,8 @@ __global__ __launch_bounds__(kVecBS) void cg_update_xp(int64_t n, X* __restrict_
                                                        const X* __restrict__ r,
                                                        const CgState<Acc>* st, int32_t it)
 {
-  if (st->k != it)
+  (void)it;
+  if (!st->xp_pending)
     return;
   const X alpha = static_cast<X>(st->alpha);
   const X beta = static_cast<X>(st->beta);
