@@ -66,20 +66,46 @@ def run_cg_section(torch, S, P, O, n, seed, cpu_iters):
         return torch.linalg.norm(b - r).item() / bnorm
 
     out = {"workload": f"7-point Poisson {n}^3, FP64 CSR ({A.nnz} nnz), b uniform[-1,1] seed {seed}",
-           "rtol": 1e-10}
-    # warm up the workspaces once (not timed)
+           "rtol": 1e-10,
+           "baseline": "fp64: plain FP64 CG on the CSR arrays (the reference's algorithm)"}
+    try:
+        info = M64.sliced_info()
+        out["sliced_layout"] = {"patterns": info["npatterns"], "longest_row": info["wmax"],
+                                "fp64_value_slots": info["slots"],
+                                "bf16_value_slots": M16.sliced_info()["slots"]}
+    except S.SpmvError as e:
+        out["sliced_layout"] = {"unavailable": str(e)}
+    # warm up the workspaces and captured graphs once (not timed)
     S.cg(M64, b, 2, 1e-10)
-    S.ir_cg(M64, M32, b, rtol=1e-2, inner_rtol=1e-1, max_outer=1)
-    S.ir_cg(M64, M16, b, rtol=1e-2, inner_rtol=1e-1, max_outer=1)
+    for Ml in (M32, M16):
+        S.ir_cg(M64, Ml, b, rtol=1e-2, inner_rtol=1e-1, max_outer=1)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    x, k = S.cg(M64, b, 100000, 1e-10)
+    x64, k = S.cg(M64, b, 100000, 1e-10)
     torch.cuda.synchronize()
     t64 = time.perf_counter() - t
-    out["fp64"] = {"seconds": t64, "iterations": k, "true_rel_residual": true_rel(x),
+    x64n = torch.linalg.norm(x64).item()
+    out["fp64"] = {"seconds": t64, "iterations": k, "true_rel_residual": true_rel(x64),
                    "ms_per_iteration": 1e3 * t64 / max(k, 1)}
     log(f"[cg] fp64: {t64:.3f}s {k} its")
-    for name, Ml in (("fp32_ir", M32), ("bf16_ir", M16)):
+    runs = [("fp32_ir", M32, -1), ("bf16_ir", M16, -1)]
+    if "patterns" in out["sliced_layout"]:
+        runs += [("fp32_ir_sliced", M32, S.SLICED_VARIANT), ("bf16_ir_sliced", M16, S.SLICED_VARIANT)]
+        M64.set_kernel(S.SLICED_VARIANT)
+        S.cg(M64, b, 2, 1e-10)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        xs, ks = S.cg(M64, b, 100000, 1e-10)
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t
+        out["fp64_sliced"] = {"seconds": ts, "iterations": ks, "true_rel_residual": true_rel(xs),
+                              "ms_per_iteration": 1e3 * ts / max(ks, 1),
+                              "speedup_vs_fp64": t64 / ts}
+        log(f"[cg] fp64_sliced: {ts:.3f}s {ks} its")
+        M64.set_kernel(-1)
+    for name, Ml, var in runs:
+        Ml.set_kernel(var)
+        S.ir_cg(M64, Ml, b, rtol=1e-2, inner_rtol=1e-1, max_outer=1)  # layout + graph warm-up
         torch.cuda.synchronize()
         t = time.perf_counter()
         x, st = S.ir_cg(M64, Ml, b, rtol=1e-10, inner_rtol=1e-4)
@@ -87,8 +113,10 @@ def run_cg_section(torch, S, P, O, n, seed, cpu_iters):
         tm = time.perf_counter() - t
         out[name] = {"seconds": tm, "outer": st["outer_its"], "inner_iterations": st["inner_its"],
                      "inner_per_outer": st["inner_hist"], "true_rel_residual": true_rel(x),
+                     "rel_err_vs_fp64_cg": torch.linalg.norm(x - x64).item() / x64n,
                      "speedup_vs_fp64": t64 / tm}
         log(f"[cg] {name}: {tm:.3f}s outer {st['outer_its']} inner {st['inner_its']}")
+        Ml.set_kernel(-1)
     if cpu_iters > 0:
         oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
         bh = b.cpu().numpy()

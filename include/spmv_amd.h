@@ -159,9 +159,23 @@ int spmvx_matrix_arrays(spmvx_matrix A, const int32_t** rowptr_dev,
 /* Selects the SpMV kernel variant of A (staging chunk, loads in flight,
  * streaming loads; 0 <= variant < spmvx_num_kernels(), -1 = default).  All
  * variants give identical results; used by tuning.  No reference
- * counterpart (the inspector of Matrix.cpp:43 picks its own kernel). */
+ * counterpart (the inspector of Matrix.cpp:43 picks its own kernel).
+ * The last variant, SPMVX_SLICED_VARIANT, is the sliced row-pattern layout:
+ * for matrices whose rows repeat a few column-offset patterns (stencils), a
+ * copy of the values is kept in 64-row slices with a 16-bit pattern id per
+ * row instead of column indices (the CSR arrays stay as they are).  Selecting
+ * it builds that copy; a matrix that does not fit the layout (more than
+ * 65535 patterns, rows over 256 entries, a pattern table over 8192 entries)
+ * returns SPMVX_ERR_ARG with the reason and keeps its variant.  Transposed
+ * products use the default CSR variant. */
+#define SPMVX_SLICED_VARIANT 11
 int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant);
 int spmvx_num_kernels(void);
+/* Row-pattern analysis of A for the sliced layout (built if needed; no
+ * values are copied): number of distinct patterns, longest row and value
+ * slots of the layout for A's dtype (nnz plus padding).  SPMVX_ERR_ARG with
+ * the reason when A does not fit the layout.  No reference counterpart. */
+int spmvx_matrix_sliced_info(spmvx_matrix A, int32_t* npatterns, int32_t* wmax, int64_t* slots);
 /* Overwrite the values from nnz doubles on the device (pattern unchanged). */
 int spmvx_matrix_set_values(spmvx_matrix A, const double* values_dev, spmvx_stream stream);
 

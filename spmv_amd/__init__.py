@@ -24,9 +24,11 @@ import os
 import numpy as np
 
 __all__ = ["L2GMap", "Matrix", "Batch", "cg", "cg_f32", "ir_cg", "sddmm", "copy_f64", "SpmvError",
-           "F64", "F32", "BF16", "lib", "library_path"]
+           "F64", "F32", "BF16", "SLICED_VARIANT", "lib", "library_path"]
 
 F64, F32, BF16 = 0, 1, 2
+# kernel variant of the sliced row-pattern layout (SPMVX_SLICED_VARIANT)
+SLICED_VARIANT = 11
 _DTYPES = {"f64": F64, "float64": F64, "double": F64, F64: F64,
            "f32": F32, "float32": F32, "float": F32, F32: F32,
            "bf16": BF16, "bfloat16": BF16, BF16: BF16}
@@ -82,6 +84,7 @@ def lib():
         "spmvx_matrix_set_values": ([P, P, P], ctypes.c_int),
         "spmvx_matrix_set_kernel": ([P, i32], ctypes.c_int),
         "spmvx_num_kernels": ([], ctypes.c_int),
+        "spmvx_matrix_sliced_info": ([P, P, P, P], ctypes.c_int),
         "spmvx_spmv": ([P, P, P, P], ctypes.c_int),
         "spmvx_spmv_t": ([P, P, P, P], ctypes.c_int),
         "spmvx_cg": ([P, P, P, P, i32, f64, P, P, P], ctypes.c_int),
@@ -354,8 +357,18 @@ class Matrix:
         return out
 
     def set_kernel(self, variant):
-        """SpMV kernel variant (-1 = default); results are identical."""
+        """SpMV kernel variant (-1 = default); results are identical.
+        ``SLICED_VARIANT`` selects the sliced row-pattern layout and raises
+        SpmvError with the reason when the matrix does not fit it."""
         _check(lib().spmvx_matrix_set_kernel(self._h, int(variant)))
+
+    def sliced_info(self):
+        """Row-pattern analysis for the sliced layout: dict(npatterns, wmax,
+        slots); raises SpmvError with the reason when the matrix does not fit."""
+        n, w, sl = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        _check(lib().spmvx_matrix_sliced_info(self._h, ctypes.byref(n), ctypes.byref(w),
+                                              ctypes.byref(sl)))
+        return {"npatterns": n.value, "wmax": w.value, "slots": sl.value}
 
     def set_values(self, values_f64, stream=None):
         _check(lib().spmvx_matrix_set_values(self._h, _dptr(values_f64), _stream(stream)))

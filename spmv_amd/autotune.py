@@ -3,7 +3,7 @@
 * ``tune_batch``  -- workgroup size and lanes per row (rows per wavefront =
   64 / lanes) of the persistent batched CG kernel, timed on the batch itself;
 * ``tune_spmv``   -- SpMV kernel variant (staging chunk, vector width, loads
-  in flight, pipelining) of a matrix.
+  in flight, pipelining, or the sliced row-pattern layout) of a matrix.
 
 Results are cached in a JSON file keyed by a signature of the problem
 (``$SPMV_AMD_TUNE_CACHE``, default ``~/.cache/spmv_amd/tune.json``), so a
@@ -110,10 +110,14 @@ def tune_spmv(A, x, reps=10, use_cache=True):
         if hit:
             A.set_kernel(hit["variant"])
             return hit, None
+    from . import SpmvError
     y = torch.empty(A.nrows, dtype=A.vector_dtype(), device=x.device)
     table = []
     for v in range(lib().spmvx_num_kernels()):
-        A.set_kernel(v)
+        try:
+            A.set_kernel(v)
+        except SpmvError:  # a layout the matrix does not fit (sliced row patterns)
+            continue
         A.mult(x, out=y)
         ms = _time_ms(lambda: A.mult(x, out=y), reps)
         table.append({"variant": v, "ms": ms})

@@ -102,14 +102,21 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
   hipStream_t q = w.stream;
   if (col_map)
     l2gmap_prepare_device(col_map);
+  const bool sliced = A->variant == kSlicedVariant;
+  const SlicedView sv = sliced ? sliced_view(A) : SlicedView{};
+  const void* vals = sliced ? A->svalues.ptr : A->values.ptr;
 
   // one CG iteration (cg.cpp:52-80); every kernel is gated by the state
   auto iteration = [&](hipStream_t qq) {
     if (col_map)
       l2gmap_update_async(col_map, vec_dtype, p, qq); // cg.cpp:55
-    const int64_t nparts = launch_csr_stream<V, X, Acc, kDot>(
-        A->variant, rb.nblocks, qq, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
-        static_cast<const V*>(A->values.ptr), p, y, rb.desc.as<int2>(), nullptr, part_s, st, 1);
+    const int64_t nparts =
+        sliced ? launch_sliced<V, X, Acc, kDot>(sv, qq, static_cast<const V*>(vals), p, y,
+                                                nullptr, part_s, st, 1)
+               : launch_csr_stream<V, X, Acc, kDot>(
+                     A->variant, rb.nblocks, qq, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
+                     static_cast<const V*>(vals), p, y, rb.desc.as<int2>(), nullptr, part_s,
+                     st, 1);
     cg_alpha_finalize<Acc><<<1, kFinBS, 0, qq>>>(part_s, nparts, st, 1);
     cg_update_r<X, Acc><<<vg, kVecBS, 0, qq>>>(n, r, y, part_1, st, 1);
     cg_beta_finalize<Acc><<<1, kFinBS, 0, qq>>>(part_1, vg, st, hist_dev, 1);
@@ -120,7 +127,7 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
   const int32_t halo_dt = col_map ? col_map->halo_dtype : -2;
   if (!w.exec || w.g_x != x || w.g_hist != hist_dev || w.g_map != col_map || w.g_halo != halo
       || w.g_halo_dtype != halo_dt || w.g_variant != A->variant || w.g_mem != w.mem.ptr
-      || w.g_values != A->values.ptr)
+      || w.g_values != vals)
   {
     if (w.exec)
       SPMVX_HIP(hipGraphExecDestroy(w.exec));
@@ -144,7 +151,7 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
     w.g_halo_dtype = halo_dt;
     w.g_variant = A->variant;
     w.g_mem = w.mem.ptr;
-    w.g_values = A->values.ptr;
+    w.g_values = vals;
   }
 
   // the caller's stream first (b and x may have just been written on it)
@@ -309,10 +316,15 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
     for (;;)
     {
       // r = b - A x and ||r||^2 in FP64
-      const int64_t nparts = launch_csr_stream<double, double, double, kResid>(
-          A64->variant, rb64.nblocks, s, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
-          static_cast<const double*>(A64->values.ptr), x, r, rb64.desc.as<int2>(), b,
-          static_cast<double*>(w.red_spmv), st, 0);
+      const int64_t nparts =
+          A64->variant == kSlicedVariant
+              ? launch_sliced<double, double, double, kResid>(
+                    sliced_view(A64), s, A64->svalues.as<double>(), x, r, b,
+                    static_cast<double*>(w.red_spmv), st, 0)
+              : launch_csr_stream<double, double, double, kResid>(
+                    A64->variant, rb64.nblocks, s, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
+                    static_cast<const double*>(A64->values.ptr), x, r, rb64.desc.as<int2>(), b,
+                    static_cast<double*>(w.red_spmv), st, 0);
       sum_finalize<double><<<1, kFinBS, 0, s>>>(static_cast<double*>(w.red_spmv), nparts, st);
       check_launch("csr_stream<kResid>");
       SPMVX_HIP(hipMemcpyAsync(flag, st, sizeof(CgState<double>), hipMemcpyDeviceToHost, s));

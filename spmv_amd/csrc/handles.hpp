@@ -3,6 +3,7 @@
 
 #include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "kernels.hpp"
@@ -83,6 +84,31 @@ struct RowBlocks
 // CSR pattern + row-binning analysis on the device; shared between a matrix
 // and its reduced-precision copies.  The host row pointer is kept so that
 // blocks for another chunk size can be built on demand.
+// Row patterns of a CSR pattern (sliced layout): the 16-bit pattern id of
+// every row and the pattern table (length, offsets col - row) in rows of
+// wmax + 1 entries.
+struct RowPatterns
+{
+  int32_t npat = 0, wmax = 0, dict_n = 0;
+  DevBuf pid, dict;
+  std::vector<uint16_t> h_pid;
+  std::vector<int32_t> h_len; // length of each pattern
+};
+
+// Slice descriptors (first value slot / 64, padded width) for one number
+// of entries per lane and load (VW).
+struct SliceLayout
+{
+  int64_t nslices = 0, slots = 0;
+  DevBuf sdesc;
+};
+
+// Limits of the sliced layout: distinct patterns (16-bit ids), the longest
+// row, and the pattern table held in LDS (32 KiB).
+constexpr int32_t kMaxPatterns = 65535;
+constexpr int32_t kMaxSliceWidth = 256;
+constexpr int32_t kMaxDictEntries = 8192;
+
 struct Pattern
 {
   int64_t nrows = 0, ncols = 0, nnz = 0;
@@ -91,6 +117,13 @@ struct Pattern
   std::map<int, std::unique_ptr<RowBlocks>> blocks; // by chunk
   int device = 0;
   const RowBlocks& blocks_for(int chunk);
+  // sliced layout, analysed on first use; null with a reason if not eligible
+  std::unique_ptr<RowPatterns> rpat;
+  std::string rpat_error;
+  bool rpat_tried = false;
+  std::map<int, std::unique_ptr<SliceLayout>> slices; // by VW
+  const RowPatterns* row_patterns();
+  const SliceLayout& slice_layout(int vw);
 };
 
 // Row blocks for the ordered stream kernel: consecutive rows, at most
@@ -101,6 +134,10 @@ std::vector<int2> row_blocks(const int32_t* rowptr, int64_t nrows, int64_t max_n
 
 // Default kernel variant of a value type (overridable with SPMVX_SPMV_VARIANT).
 int default_variant(int32_t dtype);
+
+// Builds the sliced layout of a matrix (row patterns, slices, values) if it
+// is missing; throws with the reason when the pattern is not eligible.
+SlicedView sliced_view(spmvx_matrix_s* A);
 
 // Grid of the vector kernels for n elements.
 inline int vec_grid(int64_t n)
@@ -174,6 +211,9 @@ struct spmvx_matrix_s
   // transposed copy for spmv_t (built on first use)
   std::shared_ptr<spmvx::Pattern> tpat;
   spmvx::DevBuf tvalues;
+  // values in the sliced row-pattern layout (built when that kernel is chosen)
+  spmvx::DevBuf svalues;
+  bool svalues_ready = false;
   // solver scratch (FP64 / FP32)
   spmvx::CgWorkspace ws64, ws32;
   spmvx::PinnedBuf host_flag;

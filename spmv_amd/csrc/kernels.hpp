@@ -554,8 +554,12 @@ constexpr SpmvVariant kVariants[kNumVariants] = {SPMVX_VARIANTS(SPMVX_VARIANT_EN
 #undef SPMVX_VARIANT_ENTRY
 
 // Entries per row block of a variant (pipelined kernels keep 8 spare slots).
+// The sliced layout (kSlicedVariant, defined below) reports the blocks of
+// variant 0, which CSR-only operations (transpose, SDDMM) then use.
 inline int variant_block_nnz(int var)
 {
+  if (var < 0 || var >= kNumVariants)
+    var = 0;
   return kVariants[var].kind == 1 ? kVariants[var].chunk - 8 : kVariants[var].chunk;
 }
 
@@ -595,6 +599,145 @@ int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t
   }
 #undef SPMVX_CASE
   return grid;
+}
+
+// ---------------------------------------------------------------- sliced row patterns
+// Layout for matrices whose rows repeat a few column-offset patterns
+// (stencils, structured meshes): no column indices are stored.  Each row keeps
+// the 16-bit id of its pattern (the offsets col - row of its entries, in the
+// row's CSR order) and its values; the values of 64 consecutive rows (a
+// slice, one wavefront) are interleaved so that entry k of all 64 rows is one
+// coalesced load: element (row lane, entry k) of a slice sits at
+//   slice_base + (k / VW) * 64 * VW + lane * VW + k % VW,
+// the slice width (longest row) rounded up to VW and padded with zeros.
+// Patterns live in LDS as rows of (length, offset_0 .. offset_{wmax-1}).
+// Entries are summed in CSR order from 0 with separately rounded products, so
+// every y entry is bit-identical to csr_stream's.
+constexpr int kSlice = 64;
+constexpr int kSliceBS = 256;  // four slices per workgroup
+constexpr int kSlicedVariant = SPMVX_SLICED_VARIANT;  // kernel variant id of this layout
+static_assert(kSlicedVariant == kNumVariants, "the sliced layout follows the CSR variants");
+
+// Entries per lane per value load: 8-byte loads for every value type.
+template <typename V>
+constexpr int sliced_vw()
+{
+  return static_cast<int>(8 / sizeof(V));
+}
+
+template <typename V, typename X, typename Acc, int MODE, int UNR>
+__global__ __launch_bounds__(kSliceBS) void sliced_spmv(
+    const uint16_t* __restrict__ pid, const int2* __restrict__ sdesc, int64_t nslices,
+    int64_t nrows, int64_t ncols, const int32_t* __restrict__ dict, int32_t dict_n,
+    int32_t wmax, const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
+    const X* __restrict__ b, Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it)
+{
+  if (MODE != kPlain && !cg_live(st, it))
+    return;
+  constexpr int VW = sliced_vw<V>();
+  extern __shared__ int32_t sdict[];  // dict_n entries
+  __shared__ Acc red[16];
+  for (int i = threadIdx.x; i < dict_n; i += kSliceBS)
+    sdict[i] = dict[i];
+  const int64_t nblk = (nslices + 3) / 4;
+  const int64_t bi = xcd_block(blockIdx.x, nblk);
+  const int64_t slice = bi * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int64_t row = slice * kSlice + lane;
+  __syncthreads();
+  Acc part = Acc(0);
+  if (slice < nslices)  // uniform per wavefront
+  {
+    const int2 d = sdesc[slice];
+    const int W = d.y;
+    const bool has_row = row < nrows;
+    const int p = has_row ? static_cast<int>(pid[row]) : 0;
+    const int32_t* pat = sdict + p * (wmax + 1);
+    const int len = has_row ? pat[0] : 0;
+    // column used by idle slots: any valid index (ncols >= 1 when nnz >= 1)
+    const int32_t rc = static_cast<int32_t>(row < ncols ? row : ncols - 1);
+    const int32_t r32 = static_cast<int32_t>(row);
+    const V* vs = val + static_cast<int64_t>(d.x) * kSlice + lane * VW;
+    Acc s = Acc(0);
+    for (int k0 = 0; k0 < W; k0 += VW * UNR)
+    {
+      V v[UNR][VW];
+      X xv[UNR][VW];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u)
+      {
+        const int kk = k0 + u * VW;
+        const int kl = kk < W ? kk : 0;
+        load_vec<true>(vs + (kl / VW) * (kSlice * VW), v[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u)
+#pragma unroll
+        for (int j = 0; j < VW; ++j)
+        {
+          const int k = k0 + u * VW + j;
+          const bool ok = k < len;
+          const int32_t o = pat[1 + (ok ? k : 0)];  // stays inside the pattern row
+          xv[u][j] = x[ok ? r32 + o : rc];
+        }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u)
+#pragma unroll
+        for (int j = 0; j < VW; ++j)
+        {
+          const int k = k0 + u * VW + j;
+          const Acc pr = static_cast<Acc>(widen(v[u][j])) * static_cast<Acc>(xv[u][j]);
+          s = (k < len) ? s + pr : s;
+        }
+    }
+    if (has_row)
+    {
+      if (MODE == kResid)
+      {
+        const X yi = b[row] - static_cast<X>(s);
+        y[row] = yi;
+        part = static_cast<Acc>(yi) * static_cast<Acc>(yi);
+      }
+      else
+      {
+        const X yi = static_cast<X>(s);
+        y[row] = yi;
+        if (MODE == kDot)
+          part = static_cast<Acc>(x[row]) * static_cast<Acc>(yi);
+      }
+    }
+  }
+  if (MODE == kPlain)
+    return;
+  part = block_sum(part, red);
+  if (threadIdx.x == 0)
+    partials[bi] = part;
+}
+
+// Device arrays of a sliced layout (values excluded).
+struct SlicedView
+{
+  const uint16_t* pid;
+  const int2* sdesc;
+  int64_t nslices, nrows, ncols;
+  const int32_t* dict;
+  int32_t dict_n, wmax;
+};
+
+// Launches the sliced kernel; returns the number of partial sums written.
+template <typename V, typename X, typename Acc, int MODE>
+int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X* x, X* y,
+                      const X* b, Acc* partials, const CgState<Acc>* st, int32_t it)
+{
+  const int64_t nblk = (S.nslices + 3) / 4;
+  if (nblk <= 0)
+    return 0;
+  constexpr int UNR = sizeof(V) == 8 ? 4 : 2;
+  sliced_spmv<V, X, Acc, MODE, UNR><<<dim3(static_cast<unsigned>(nblk)), dim3(kSliceBS),
+                                      sizeof(int32_t) * static_cast<size_t>(S.dict_n), s>>>(
+      S.pid, S.sdesc, S.nslices, S.nrows, S.ncols, S.dict, S.dict_n, S.wmax, val, x, y, b,
+      partials, st, it);
+  return nblk;
 }
 
 // ---------------------------------------------------------------- CG vectors

@@ -3,6 +3,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <string>
+#include <unordered_map>
 
 #include "handles.hpp"
 
@@ -47,6 +50,120 @@ const RowBlocks& Pattern::blocks_for(int chunk)
   SPMVX_HIP(hipMemcpy(rb->desc.ptr, b.data(), sizeof(int2) * b.size(), hipMemcpyHostToDevice));
   const RowBlocks& ref = *rb;
   blocks[chunk] = std::move(rb);
+  return ref;
+}
+
+const RowPatterns* Pattern::row_patterns()
+{
+  if (rpat_tried)
+    return rpat.get();
+  rpat_tried = true;
+  if (nnz == 0 || nrows == 0)
+  {
+    rpat_error = "the sliced layout needs a matrix with entries";
+    return nullptr;
+  }
+  std::vector<int32_t> ci(static_cast<size_t>(nnz));
+  SPMVX_HIP(hipMemcpy(ci.data(), colidx.ptr, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+  auto R = std::make_unique<RowPatterns>();
+  R->h_pid.resize(static_cast<size_t>(nrows));
+  // pattern table under construction: offsets of pattern q at start[q]
+  std::vector<int32_t> offs, start;
+  std::unordered_map<uint64_t, std::vector<int32_t>> buckets;
+  int32_t wmax = 0;
+  for (int64_t r = 0; r < nrows; ++r)
+  {
+    const int32_t a = h_rowptr[r], e = h_rowptr[r + 1];
+    const int32_t len = e - a;
+    if (len > kMaxSliceWidth)
+    {
+      rpat_error = "a row is longer than the sliced layout allows (" + std::to_string(len) + " > "
+                   + std::to_string(kMaxSliceWidth) + ")";
+      return nullptr;
+    }
+    uint64_t h = 1469598103934665603ull ^ static_cast<uint64_t>(len);
+    for (int32_t k = a; k < e; ++k)
+      h = (h ^ static_cast<uint64_t>(static_cast<uint32_t>(ci[k] - static_cast<int32_t>(r))))
+          * 1099511628211ull;
+    std::vector<int32_t>& cand = buckets[h];
+    int32_t id = -1;
+    for (int32_t q : cand)
+      if (R->h_len[q] == len
+          && std::equal(offs.begin() + start[q], offs.begin() + start[q] + len, ci.begin() + a,
+                        [r](int32_t o, int32_t c) { return o == c - static_cast<int32_t>(r); }))
+      {
+        id = q;
+        break;
+      }
+    if (id < 0)
+    {
+      id = static_cast<int32_t>(R->h_len.size());
+      if (id >= kMaxPatterns)
+      {
+        rpat_error = "more than " + std::to_string(kMaxPatterns)
+                     + " distinct row patterns (the sliced layout suits stencil-like matrices)";
+        return nullptr;
+      }
+      cand.push_back(id);
+      start.push_back(static_cast<int32_t>(offs.size()));
+      R->h_len.push_back(len);
+      for (int32_t k = a; k < e; ++k)
+        offs.push_back(ci[k] - static_cast<int32_t>(r));
+      wmax = std::max(wmax, len);
+      if (static_cast<int64_t>(R->h_len.size()) * (wmax + 1) > kMaxDictEntries)
+      {
+        rpat_error = "the row-pattern table exceeds " + std::to_string(kMaxDictEntries)
+                     + " entries (too many or too long patterns for the sliced layout)";
+        return nullptr;
+      }
+    }
+    R->h_pid[r] = static_cast<uint16_t>(id);
+  }
+  R->npat = static_cast<int32_t>(R->h_len.size());
+  R->wmax = std::max(wmax, 1);
+  R->dict_n = R->npat * (R->wmax + 1);
+  std::vector<int32_t> dict(static_cast<size_t>(R->dict_n), 0);
+  for (int32_t q = 0; q < R->npat; ++q)
+  {
+    dict[q * (R->wmax + 1)] = R->h_len[q];
+    for (int32_t k = 0; k < R->h_len[q]; ++k)
+      dict[q * (R->wmax + 1) + 1 + k] = offs[start[q] + k];
+  }
+  R->dict.ensure(sizeof(int32_t) * dict.size());
+  SPMVX_HIP(hipMemcpy(R->dict.ptr, dict.data(), sizeof(int32_t) * dict.size(), hipMemcpyHostToDevice));
+  R->pid.ensure(sizeof(uint16_t) * static_cast<size_t>(nrows));
+  SPMVX_HIP(hipMemcpy(R->pid.ptr, R->h_pid.data(), sizeof(uint16_t) * nrows, hipMemcpyHostToDevice));
+  rpat = std::move(R);
+  return rpat.get();
+}
+
+const SliceLayout& Pattern::slice_layout(int vw)
+{
+  auto it = slices.find(vw);
+  if (it != slices.end())
+    return *it->second;
+  const RowPatterns* R = row_patterns();
+  SPMVX_REQUIRE(R != nullptr, rpat_error);
+  auto L = std::make_unique<SliceLayout>();
+  L->nslices = (nrows + kSlice - 1) / kSlice;
+  std::vector<int2> d(static_cast<size_t>(L->nslices));
+  int64_t off64 = 0;
+  for (int64_t sl = 0; sl < L->nslices; ++sl)
+  {
+    int32_t w = 0;
+    const int64_t r1 = std::min<int64_t>(nrows, (sl + 1) * kSlice);
+    for (int64_t r = sl * kSlice; r < r1; ++r)
+      w = std::max(w, R->h_len[R->h_pid[r]]);
+    w = (w + vw - 1) / vw * vw;
+    SPMVX_REQUIRE(off64 < (int64_t(1) << 31), "sliced layout too large for 32-bit slice offsets");
+    d[sl] = make_int2(static_cast<int32_t>(off64), w);
+    off64 += w;
+  }
+  L->slots = off64 * kSlice;
+  L->sdesc.ensure(sizeof(int2) * d.size());
+  SPMVX_HIP(hipMemcpy(L->sdesc.ptr, d.data(), sizeof(int2) * d.size(), hipMemcpyHostToDevice));
+  const SliceLayout& ref = *L;
+  slices[vw] = std::move(L);
   return ref;
 }
 
@@ -166,6 +283,29 @@ __global__ void copy_f64_kernel(double* __restrict__ dst, const double* __restri
     dst[n - 1] = src[n - 1];
 }
 
+// dst[slot(row, k)] = src[rowptr[row] + k]: CSR values into the sliced
+// row-pattern layout (padding slots keep the zeros they were allocated with).
+template <typename T>
+__global__ void scatter_sliced(const int32_t* __restrict__ rowptr,
+                               const uint16_t* __restrict__ pid,
+                               const int32_t* __restrict__ dict, int32_t wmax,
+                               const int2* __restrict__ sdesc, int64_t nrows, int vw,
+                               const T* __restrict__ src, T* __restrict__ dst)
+{
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t r = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; r < nrows;
+       r += stride)
+  {
+    const int64_t sl = r / kSlice;
+    const int lane = static_cast<int>(r % kSlice);
+    const int64_t base = static_cast<int64_t>(sdesc[sl].x) * kSlice;
+    const int len = dict[static_cast<int64_t>(pid[r]) * (wmax + 1)];
+    const int32_t a = rowptr[r];
+    for (int k = 0; k < len; ++k)
+      dst[base + (k / vw) * (kSlice * vw) + lane * vw + k % vw] = src[a + k];
+  }
+}
+
 // out[k] = alpha * u[row(k)] * v[col(k)]; one workgroup per row block, rows of
 // an entry found by binary search in the block's row pointers (LDS).
 __global__ __launch_bounds__(kBS) void sddmm_kernel(const int32_t* __restrict__ rowptr,
@@ -219,6 +359,28 @@ void launch_spmv(int32_t dtype, int var, Pattern& P, const void* vals, const voi
     launch_plain<float, float, float>(var, P, vals, x, y, s);
   else
     launch_plain<bf16_t, float, float>(var, P, vals, x, y, s);
+}
+
+// Fills A->svalues from A->values on stream s (layout built beforehand).
+void scatter_values(spmvx_matrix_s* A, const RowPatterns& R, const SliceLayout& L, hipStream_t s)
+{
+  const Pattern& P = *A->pat;
+  const size_t es = dtype_size(A->dtype);
+  const int vw = static_cast<int>(8 / es);
+  const int64_t g64 = std::min<int64_t>((P.nrows + 255) / 256, 8192);
+  const dim3 grid(static_cast<unsigned>(g64 > 0 ? g64 : 1)), block(256);
+#define SPMVX_SCATTER(T)                                                                    \
+  scatter_sliced<T><<<grid, block, 0, s>>>(P.rowptr.as<int32_t>(), R.pid.as<uint16_t>(),    \
+                                           R.dict.as<int32_t>(), R.wmax, L.sdesc.as<int2>(), \
+                                           P.nrows, vw, A->values.as<T>(), A->svalues.as<T>())
+  if (es == 8)
+    SPMVX_SCATTER(uint64_t);
+  else if (es == 4)
+    SPMVX_SCATTER(uint32_t);
+  else
+    SPMVX_SCATTER(uint16_t);
+#undef SPMVX_SCATTER
+  check_launch("scatter_sliced");
 }
 
 std::shared_ptr<Pattern> upload_pattern(int64_t nrows, int64_t ncols, int64_t nnz,
@@ -287,6 +449,63 @@ void build_transpose(spmvx_matrix_s* A)
   A->tvalues.ensure(es * static_cast<size_t>(P.nnz + 16));
   if (P.nnz)
     SPMVX_HIP(hipMemcpy(A->tvalues.ptr, tv.data(), es * P.nnz, hipMemcpyHostToDevice));
+}
+
+} // namespace
+
+SlicedView sliced_view(spmvx_matrix_s* A)
+{
+  Pattern& P = *A->pat;
+  const int vw = static_cast<int>(8 / dtype_size(A->dtype));
+  const RowPatterns* R = P.row_patterns();
+  SPMVX_REQUIRE(R != nullptr, P.rpat_error);
+  const SliceLayout& L = P.slice_layout(vw);
+  if (!A->svalues_ready)
+  {
+    A->svalues.release();
+    A->svalues.ensure(dtype_size(A->dtype) * static_cast<size_t>(L.slots + 16));
+    scatter_values(A, *R, L, nullptr);
+    SPMVX_HIP(hipDeviceSynchronize());
+    A->svalues_ready = true;
+  }
+  return SlicedView{R->pid.as<uint16_t>(), L.sdesc.as<int2>(), L.nslices, P.nrows, P.ncols,
+                    R->dict.as<int32_t>(),  R->dict_n,          R->wmax};
+}
+
+namespace
+{
+
+// CSR kernel variant used where the sliced layout does not apply
+// (transposed products).
+int csr_variant(const spmvx_matrix_s* A)
+{
+  return A->variant == kSlicedVariant ? default_variant(A->dtype) : A->variant;
+}
+
+void launch_matrix_spmv(spmvx_matrix_s* A, const void* x, void* y, hipStream_t s)
+{
+  if (A->variant != kSlicedVariant)
+  {
+    launch_spmv(A->dtype, A->variant, *A->pat, A->values.ptr, x, y, s);
+    return;
+  }
+  const SlicedView v = sliced_view(A);
+  if (A->dtype == SPMVX_F64)
+    launch_sliced<double, double, double, kPlain>(v, s, A->svalues.as<double>(),
+                                                  static_cast<const double*>(x),
+                                                  static_cast<double*>(y), nullptr, nullptr,
+                                                  nullptr, 0);
+  else if (A->dtype == SPMVX_F32)
+    launch_sliced<float, float, float, kPlain>(v, s, A->svalues.as<float>(),
+                                               static_cast<const float*>(x),
+                                               static_cast<float*>(y), nullptr, nullptr, nullptr,
+                                               0);
+  else
+    launch_sliced<bf16_t, float, float, kPlain>(v, s, A->svalues.as<bf16_t>(),
+                                                static_cast<const float*>(x),
+                                                static_cast<float*>(y), nullptr, nullptr, nullptr,
+                                                0);
+  check_launch("sliced SpMV");
 }
 
 } // namespace
@@ -394,6 +613,11 @@ int spmvx_matrix_set_values(spmvx_matrix A, const double* values_dev, spmvx_stre
     {
       convert_values<<<1024, 256, 0, as_stream(stream)>>>(values_dev, A->values.ptr, nnz, A->dtype);
       check_launch("convert_values");
+      if (A->svalues_ready)
+      {
+        const int vw = static_cast<int>(8 / dtype_size(A->dtype));
+        scatter_values(A, *A->pat->row_patterns(), A->pat->slice_layout(vw), as_stream(stream));
+      }
     }
     A->tpat.reset(); // transposed copy is stale
     A->tvalues.release();
@@ -404,12 +628,28 @@ int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant)
 {
   return guarded([&] {
     SPMVX_REQUIRE(A != nullptr, "NULL matrix");
-    SPMVX_REQUIRE(variant >= -1 && variant < kNumVariants, "unknown kernel variant");
+    SPMVX_REQUIRE(variant >= -1 && variant <= kSlicedVariant, "unknown kernel variant");
+    if (variant == kSlicedVariant)
+      (void)sliced_view(A); // builds the layout or reports why it does not apply
     A->variant = variant < 0 ? default_variant(A->dtype) : variant;
   });
 }
 
-int spmvx_num_kernels(void) { return kNumVariants; }
+int spmvx_num_kernels(void) { return kSlicedVariant + 1; }
+
+int spmvx_matrix_sliced_info(spmvx_matrix A, int32_t* npatterns, int32_t* wmax, int64_t* slots)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    Pattern& P = *A->pat;
+    const RowPatterns* R = P.row_patterns();
+    SPMVX_REQUIRE(R != nullptr, P.rpat_error);
+    const SliceLayout& L = P.slice_layout(static_cast<int>(8 / dtype_size(A->dtype)));
+    if (npatterns) *npatterns = R->npat;
+    if (wmax) *wmax = R->wmax;
+    if (slots) *slots = L.slots;
+  });
+}
 
 int spmvx_spmv(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
 {
@@ -417,7 +657,7 @@ int spmvx_spmv(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
     SPMVX_REQUIRE(A != nullptr, "NULL matrix");
     SPMVX_REQUIRE(A->pat->nrows == 0 || y != nullptr, "y is NULL");
     SPMVX_REQUIRE(A->pat->nnz == 0 || x != nullptr, "x is NULL");
-    launch_spmv(A->dtype, A->variant, *A->pat, A->values.ptr, x, y, as_stream(stream));
+    launch_matrix_spmv(A, x, y, as_stream(stream));
   });
 }
 
@@ -428,7 +668,7 @@ int spmvx_spmv_t(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
     if (!A->tpat)
       build_transpose(A);
     SPMVX_REQUIRE(A->pat->ncols == 0 || y != nullptr, "y is NULL");
-    launch_spmv(A->dtype, A->variant, *A->tpat, A->tvalues.ptr, x, y, as_stream(stream));
+    launch_spmv(A->dtype, csr_variant(A), *A->tpat, A->tvalues.ptr, x, y, as_stream(stream));
   });
 }
 

@@ -25,8 +25,9 @@ def _ocsr(A):
     return O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
 
 
-def check_cg(A, b, kmax, rtol, col_map=None, halo=None, hist_tol=1e-10):
+def check_cg(A, b, kmax, rtol, col_map=None, halo=None, hist_tol=1e-10, variant=-1):
     M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols, col_map=col_map)
+    M.set_kernel(variant)
     x, k, hist = S.cg(M, _dev(b), kmax, rtol, history=True)
     xo, ko, histo = O.cg(_ocsr(A), b, kmax, rtol, halo=halo)
     assert abs(k - ko) <= 1
@@ -131,3 +132,31 @@ def test_cg_f32_inner_solver():
     xo, ko, _ = O.cg_f32(_ocsr(A), A.val.astype(np.float32), b, 5000, 1e-5)
     assert abs(k - ko) <= 3
     assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-4 * np.linalg.norm(xo)
+
+
+# ---------------------------------------------------------------- sliced layout
+@pytest.mark.parametrize("name,A,rtol", [
+    ("poisson7_24", P.stencil(7, 24, 24, 24), 1e-10),
+    ("stencil27_20", P.stencil(27, 20, 20, 20), 1e-10),
+])
+def test_cg_sliced_layout_matches_oracle(name, A, rtol):
+    xt = P.uniform(3, A.ncols, -1, 1)
+    b = O.spmv(_ocsr(A), xt)
+    check_cg(A, b, 10000, rtol, variant=S.SLICED_VARIANT)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+def test_refinement_sliced_inner(dtype):
+    A = P.stencil(7, 32, 32, 32)
+    b = P.uniform(12, A.nrows, -1, 1)
+    M64 = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    Ml = M64.like(dtype)
+    x1, st1 = S.ir_cg(M64, Ml, _dev(b), rtol=1e-10, inner_rtol=1e-4)
+    Ml.set_kernel(S.SLICED_VARIANT)
+    M64.set_kernel(S.SLICED_VARIANT)
+    x2, st2 = S.ir_cg(M64, Ml, _dev(b), rtol=1e-10, inner_rtol=1e-4)
+    xg = x2.cpu().numpy()
+    rel = np.linalg.norm(b - O.spmv(_ocsr(A), xg)) / np.linalg.norm(b)
+    assert rel <= 1e-10
+    assert st2["outer_its"] == st1["outer_its"]
+    assert abs(st2["inner_its"] - st1["inner_its"]) <= 3 * st1["outer_its"]

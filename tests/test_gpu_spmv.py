@@ -192,6 +192,8 @@ def test_all_kernel_variants_bitexact(name, A):
     xf = x.float()
     ref_b = None
     for v in range(S.lib().spmvx_num_kernels()):
+        if v == S.SLICED_VARIANT and name != "stencil27_ghost":
+            continue  # no fit: covered by test_sliced_layout_rejects
         M.set_kernel(v)
         assert np.array_equal(M.mult(x).cpu().numpy(), want), v
         Mb.set_kernel(v)
@@ -199,3 +201,71 @@ def test_all_kernel_variants_bitexact(name, A):
         if ref_b is None:
             ref_b = yb
         assert np.array_equal(yb, ref_b), v
+
+
+# ---------------------------------------------------------------- sliced row-pattern layout
+SLICED_FIT = ["demo1d", "poisson5", "poisson7", "stencil27_ghost", "diffusion", "one_row"]
+
+
+@pytest.mark.parametrize("name,A", [m for m in MATS if m[0] in SLICED_FIT], ids=SLICED_FIT)
+def test_sliced_layout_bitexact(name, A):
+    """The sliced layout sums every row in CSR order: FP64 bit-identical to
+    the oracle, FP32/BF16 bit-identical to the CSR kernels, transposed
+    products unchanged (they use the CSR arrays)."""
+    x = P.uniform(17, A.ncols, -1, 1)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    M.set_kernel(S.SLICED_VARIANT)
+    assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), O.spmv(_ocsr(A), x))
+    xt = P.uniform(18, A.nrows, -1, 1)
+    assert np.array_equal(M.transpmult(_dev(xt)).cpu().numpy(), O.spmv_t(_ocsr(A), xt))
+    xf = _dev(x, torch.float32)
+    for dt in ("f32", "bf16"):
+        Ml = M.like(dt)
+        ref = Ml.mult(xf).cpu().numpy()
+        Ml.set_kernel(S.SLICED_VARIANT)
+        assert np.array_equal(Ml.mult(xf).cpu().numpy(), ref), dt
+
+
+def test_sliced_layout_info():
+    # 7-point stencil on a 13 x 12 x 11 grid: 3 boundary classes per axis
+    A = P.stencil(7, 13, 12, 11)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    info = M.sliced_info()
+    assert info["npatterns"] == 27 and info["wmax"] == 7
+    lens = np.diff(A.rowptr)
+    nsl = -(-A.nrows // 64)
+    w = [lens[64 * i:64 * (i + 1)].max() for i in range(nsl)]
+    assert info["slots"] == 64 * sum(w)  # FP64: one entry per lane and load
+    assert M.like("bf16").sliced_info()["slots"] == 64 * sum(-(-x // 4) * 4 for x in w)
+
+
+@pytest.mark.parametrize("name,A,reason", [
+    ("random_spd", MATS[5][1], "pattern"),
+    ("long_rows", MATS[6][1], "longer"),
+], ids=["random_spd", "long_rows"])
+def test_sliced_layout_rejects(name, A, reason):
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    with pytest.raises(S.SpmvError, match=reason):
+        M.set_kernel(S.SLICED_VARIANT)
+    with pytest.raises(S.SpmvError, match=reason):
+        M.sliced_info()
+    x = P.uniform(19, A.ncols, -1, 1)
+    assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), O.spmv(_ocsr(A), x))
+
+
+def test_sliced_layout_set_values():
+    A = MATS[4][1]  # diffusion
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    Mb = M.like("bf16")
+    M.set_kernel(S.SLICED_VARIANT)
+    Mb.set_kernel(S.SLICED_VARIANT)
+    v2 = P.uniform(20, A.nnz, -2, 2)
+    M.set_values(_dev(v2))
+    Mb.set_values(_dev(v2))
+    torch.cuda.synchronize()
+    x = P.uniform(21, A.ncols, -1, 1)
+    B = P.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, v2)
+    assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), O.spmv(_ocsr(B), x))
+    Mc = S.Matrix(A.rowptr, A.colidx, v2, ncols=A.ncols, dtype="bf16")
+    xf = _dev(x, torch.float32)
+    assert np.array_equal(Mb.mult(xf).cpu().numpy(), Mc.mult(xf).cpu().numpy())
