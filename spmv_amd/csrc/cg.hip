@@ -79,7 +79,7 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
            const X* b, X* x, int32_t kmax, Acc rtol, Acc* hist_dev, hipStream_t s)
 {
   Pattern& P = *A->pat;
-  const RowBlocks& rb = P.blocks_for(variant_block_nnz(A->variant));
+  const RowBlocks& rb = variant_blocks(P, A->variant);
   ensure_ws<X, Acc>(w, P, P.ncols, rb.nblocks);
   const int64_t n = P.nrows, ncols = P.ncols;
   auto* st = static_cast<CgState<Acc>*>(w.state);
@@ -297,9 +297,9 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
     SPMVX_REQUIRE(max_outer >= 0 && max_outer <= 64, "max_outer must be in [0, 64]");
     hipStream_t s = as_stream(stream);
     const int64_t n = P.nrows;
-    const RowBlocks& rb64 = P.blocks_for(variant_block_nnz(A64->variant));
+    const RowBlocks& rb64 = variant_blocks(P, A64->variant);
     ensure_ws<double, double>(A64->ws64, P, P.ncols, rb64.nblocks);
-    const RowBlocks& rb32 = P.blocks_for(variant_block_nnz(Alow->variant));
+    const RowBlocks& rb32 = variant_blocks(P, Alow->variant);
     ensure_ws<float, float>(Alow->ws32, P, P.ncols, rb32.nblocks);
     CgWorkspace& w = A64->ws64;
     auto* st = static_cast<CgState<double>*>(w.state);

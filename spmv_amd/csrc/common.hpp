@@ -102,18 +102,36 @@ __device__ __forceinline__ T wave_sum(T v)
   return v;
 }
 
+// Workgroup barrier ordering LDS accesses only: global loads and stores
+// still in flight stay in flight (__syncthreads would wait for all of them,
+// which stalls a kernel that has prefetched its next data or just stored
+// its results).
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Block sum (blockDim.x multiple of 64, <= 1024); result valid in all
-// threads.  `scratch` holds >= 16 elements.  Deterministic order.
-template <typename T>
+// threads.  `scratch` (LDS) holds >= 16 elements.  Deterministic order.
+// LdsOnly: the barriers order LDS only (see lds_barrier).
+template <typename T, bool LdsOnly = false>
 __device__ __forceinline__ T block_sum(T v, T* scratch)
 {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
   v = wave_sum(v);
-  __syncthreads();
+  if (LdsOnly)
+    lds_barrier();
+  else
+    __syncthreads();
   if (lane == 0)
     scratch[wid] = v;
-  __syncthreads();
+  if (LdsOnly)
+    lds_barrier();
+  else
+    __syncthreads();
   T s = (lane < nw) ? scratch[lane] : T(0);
   s = wave_sum(s);
   return s;

@@ -114,9 +114,9 @@ struct Pattern
   int64_t nrows = 0, ncols = 0, nnz = 0;
   DevBuf rowptr, colidx;
   std::vector<int32_t> h_rowptr;
-  std::map<int, std::unique_ptr<RowBlocks>> blocks; // by chunk
+  std::map<std::pair<int, int>, std::unique_ptr<RowBlocks>> blocks; // by (chunk, rows)
   int device = 0;
-  const RowBlocks& blocks_for(int chunk);
+  const RowBlocks& blocks_for(int chunk, int rows = kBS);
   // sliced layout, analysed on first use; null with a reason if not eligible
   std::unique_ptr<RowPatterns> rpat;
   std::string rpat_error;
@@ -134,6 +134,12 @@ std::vector<int2> row_blocks(const int32_t* rowptr, int64_t nrows, int64_t max_n
 
 // Default kernel variant of a value type (overridable with SPMVX_SPMV_VARIANT).
 int default_variant(int32_t dtype);
+
+// Row blocks used by the CSR kernel of variant `var`.
+inline const RowBlocks& variant_blocks(Pattern& P, int var)
+{
+  return P.blocks_for(variant_block_nnz(var), variant_block_rows(var));
+}
 
 // Builds the sliced layout of a matrix (row patterns, slices, values) if it
 // is missing; throws with the reason when the pattern is not eligible.

@@ -184,10 +184,12 @@ __device__ __forceinline__ Acc row_sum(const Acc* prod, int a, int e)
 // range of a block is widened down to a multiple of VEC, the arrays are
 // padded, and entries outside the block are skipped), independent vector
 // steps in flight per thread (UNROLL), non-temporal matrix loads (NT).  Row
-// blocks hold at most kBS rows and CHUNK entries.  Block descriptors are
+// blocks hold at most BS rows (the workgroup size) and CHUNK entries.  Block
+// descriptors are
 // (first row, first entry) pairs.
-template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, int UNROLL, bool NT>
-__global__ __launch_bounds__(kBS) void csr_stream(
+template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, int UNROLL, bool NT,
+          int BS>
+__global__ __launch_bounds__(BS) void csr_stream(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
     const int2* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
@@ -195,7 +197,7 @@ __global__ __launch_bounds__(kBS) void csr_stream(
 {
   if (MODE != kPlain && !cg_live(st, it))
     return;
-  __shared__ Acc prod[CHUNK];
+  __shared__ Acc prod[CHUNK + 1];
   __shared__ Acc red[16];
 
   const int64_t bi = xcd_block(blockIdx.x, nblocks);
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(kBS) void csr_stream(
   if (k1 - k0 <= CHUNK)
   {
     const int32_t kb = k0 & ~(VEC - 1);
-    constexpr int STEP = kBS * VEC;
+    constexpr int STEP = BS * VEC;
     for (int32_t g0 = kb; g0 < k1; g0 += STEP * UNROLL)
     {
       // Branch-free loads and gathers: a vector past the block is replaced by
@@ -247,13 +249,17 @@ __global__ __launch_bounds__(kBS) void csr_stream(
 #pragma unroll
         for (int j = 0; j < VEC; ++j)
         {
+          // entries outside the block go to the spare slot prod[CHUNK]: an
+          // unconditional store keeps the compiler from sinking a gather
+          // into a branch (where it would be issued after the others return)
           const int32_t k = g + j;
-          if (g < k1 && k >= k0 && k < k1)
-            prod[k - k0] = static_cast<Acc>(widen(v[u][j])) * static_cast<Acc>(xv[u][j]);
+          const bool in = g < k1 && k >= k0 && k < k1;
+          prod[in ? k - k0 : CHUNK] =
+              static_cast<Acc>(widen(v[u][j])) * static_cast<Acc>(xv[u][j]);
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (row < r1)
     {
       const Acc s = row_sum(prod, ra - k0, re - k0);
@@ -280,13 +286,13 @@ __global__ __launch_bounds__(kBS) void csr_stream(
     for (int64_t c0 = k0; c0 < k1; c0 += CHUNK)
     {
       const int n = static_cast<int>((k1 - c0) < CHUNK ? (k1 - c0) : CHUNK);
-      __syncthreads();
-      for (int t = tid; t < n; t += kBS)
+      lds_barrier();
+      for (int t = tid; t < n; t += BS)
       {
         const int64_t k = c0 + t;
         prod[t] = static_cast<Acc>(ld_val<NT>(val, k)) * static_cast<Acc>(x[colidx[k]]);
       }
-      __syncthreads();
+      lds_barrier();
       if (tid == 0)
         for (int q = 0; q < n; ++q)
           s += prod[q];
@@ -311,7 +317,7 @@ __global__ __launch_bounds__(kBS) void csr_stream(
 
   if (MODE == kPlain)
     return;
-  part = block_sum(part, red);
+  part = block_sum<Acc, true>(part, red);
   if (tid == 0)
     partials[bi] = part;
 }
@@ -403,12 +409,13 @@ __device__ __forceinline__ void pipe_compute(const PipeStage<V, VEC, PER>& S,
 #pragma unroll
       for (int j = 0; j < VEC; ++j)
       {
-        const int32_t k = g + j;
-        if (g < k1 && k >= k0 && k < k1)
-          prod[k - k0] = static_cast<Acc>(widen(S.v[u][j])) * static_cast<Acc>(xg[u][j]);
+        const int32_t k = g + j; // outside the block: spare slot prod[CHUNK]
+        const bool in = g < k1 && k >= k0 && k < k1;
+        prod[in ? k - k0 : CHUNK] =
+            static_cast<Acc>(widen(S.v[u][j])) * static_cast<Acc>(xg[u][j]);
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (row < r1)
     {
       const Acc s = row_sum(prod, S.ra - k0, S.re - k0);
@@ -426,7 +433,7 @@ __device__ __forceinline__ void pipe_compute(const PipeStage<V, VEC, PER>& S,
           part += static_cast<Acc>(x[row]) * static_cast<Acc>(yi);
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
   else
   {
@@ -440,11 +447,11 @@ __device__ __forceinline__ void pipe_compute(const PipeStage<V, VEC, PER>& S,
         const int64_t k = c0 + t;
         prod[t] = static_cast<Acc>(ld_val<NT>(val, k)) * static_cast<Acc>(x[colidx[k]]);
       }
-      __syncthreads();
+      lds_barrier();
       if (tid == 0)
         for (int q = 0; q < n; ++q)
           s += prod[q];
-      __syncthreads();
+      lds_barrier();
     }
     if (tid == 0)
     {
@@ -477,7 +484,7 @@ __global__ __launch_bounds__(kBS) void csr_pipe(
   constexpr int PER = CHUNK / (kBS * VEC);
   static_assert(PER * kBS * VEC == CHUNK, "CHUNK must be a multiple of kBS*VEC");
   constexpr int kLong = CHUNK - 8;
-  __shared__ Acc prod[CHUNK];
+  __shared__ Acc prod[CHUNK + 1];
   __shared__ Acc red[16];
   const int64_t w = xcd_block(blockIdx.x, gridDim.x);
   const int64_t bb = nblocks * w / gridDim.x, be = nblocks * (w + 1) / gridDim.x;
@@ -521,7 +528,7 @@ __global__ __launch_bounds__(kBS) void csr_pipe(
   }
   if (MODE == kPlain)
     return;
-  part = block_sum(part, red);
+  part = block_sum<Acc, true>(part, red);
   if (threadIdx.x == 0)
     partials[w] = part;
 }
@@ -535,21 +542,25 @@ struct SpmvVariant
   int vec;
   int unroll;
   bool nt;
+  int bs; // threads per workgroup = rows per block at most
 };
 #define SPMVX_VARIANTS(X)                                                                    \
-  X(0, 0, 2048, 1, 1, false)                                                                 \
-  X(1, 0, 2048, 1, 4, true)                                                                  \
-  X(2, 0, 2048, 2, 2, true)                                                                  \
-  X(3, 0, 2048, 4, 1, true)                                                                  \
-  X(4, 0, 2048, 4, 2, true)                                                                  \
-  X(5, 1, 2048, 2, 1, true)                                                                  \
-  X(6, 1, 2048, 4, 1, true)                                                                  \
-  X(7, 1, 2048, 8, 1, true)                                                                  \
-  X(8, 1, 4096, 4, 1, true)                                                                  \
-  X(9, 1, 1024, 4, 1, true)                                                                  \
-  X(10, 1, 2048, 4, 1, false)
-constexpr int kNumVariants = 11;
-#define SPMVX_VARIANT_ENTRY(i, K, C, V, U, N) {K, C, V, U, N},
+  X(0, 0, 2048, 1, 1, false, 256)                                                            \
+  X(1, 0, 2048, 1, 4, true, 256)                                                             \
+  X(2, 0, 2048, 2, 2, true, 256)                                                             \
+  X(3, 0, 2048, 4, 1, true, 256)                                                             \
+  X(4, 0, 2048, 4, 2, true, 256)                                                             \
+  X(5, 1, 2048, 2, 1, true, 256)                                                             \
+  X(6, 1, 2048, 4, 1, true, 256)                                                             \
+  X(7, 1, 2048, 8, 1, true, 256)                                                             \
+  X(8, 1, 4096, 4, 1, true, 256)                                                             \
+  X(9, 1, 1024, 4, 1, true, 256)                                                             \
+  X(10, 1, 2048, 4, 1, false, 256)                                                           \
+  X(11, 0, 8192, 2, 4, true, 1024)                                                           \
+  X(12, 0, 8192, 2, 2, true, 1024)                                                           \
+  X(13, 0, 8192, 4, 2, true, 1024)
+constexpr int kNumVariants = 14;
+#define SPMVX_VARIANT_ENTRY(i, K, C, V, U, N, B) {K, C, V, U, N, B},
 constexpr SpmvVariant kVariants[kNumVariants] = {SPMVX_VARIANTS(SPMVX_VARIANT_ENTRY)};
 #undef SPMVX_VARIANT_ENTRY
 
@@ -561,6 +572,12 @@ inline int variant_block_nnz(int var)
   if (var < 0 || var >= kNumVariants)
     var = 0;
   return kVariants[var].kind == 1 ? kVariants[var].chunk - 8 : kVariants[var].chunk;
+}
+inline int variant_block_rows(int var)
+{
+  if (var < 0 || var >= kNumVariants)
+    var = 0;
+  return kVariants[var].bs;
 }
 
 // Launches the SpMV kernel of variant `var` over `nblocks` row blocks (built
@@ -577,11 +594,11 @@ int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t
   if (nblocks <= 0)
     return 0;
   int64_t grid = nblocks;
-#define SPMVX_CASE(i, K, C, VV, U, N)                                                        \
+#define SPMVX_CASE(i, K, C, VV, U, N, B)                                                     \
   case i:                                                                                    \
     if constexpr (K == 0)                                                                    \
-      csr_stream<V, X, Acc, MODE, C, VV, U, N><<<dim3(static_cast<unsigned>(grid)),          \
-                                                 dim3(kBS), 0, s>>>(                        \
+      csr_stream<V, X, Acc, MODE, C, VV, U, N, B><<<dim3(static_cast<unsigned>(grid)),       \
+                                                    dim3(B), 0, s>>>(                       \
           rowptr, colidx, val, x, y, blocks, nblocks, b, partials, st, it);                  \
     else                                                                                     \
     {                                                                                        \
@@ -644,7 +661,7 @@ __global__ __launch_bounds__(kSliceBS) void sliced_spmv(
   const int64_t slice = bi * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int64_t row = slice * kSlice + lane;
-  __syncthreads();
+  lds_barrier();
   Acc part = Acc(0);
   if (slice < nslices)  // uniform per wavefront
   {
@@ -709,7 +726,7 @@ __global__ __launch_bounds__(kSliceBS) void sliced_spmv(
   }
   if (MODE == kPlain)
     return;
-  part = block_sum(part, red);
+  part = block_sum<Acc, true>(part, red);
   if (threadIdx.x == 0)
     partials[bi] = part;
 }

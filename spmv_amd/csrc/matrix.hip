@@ -38,18 +38,19 @@ std::vector<int2> row_blocks(const int32_t* rowptr, int64_t nrows, int64_t max_n
   return b;
 }
 
-const RowBlocks& Pattern::blocks_for(int chunk)
+const RowBlocks& Pattern::blocks_for(int chunk, int rows)
 {
-  auto it = blocks.find(chunk);
+  const std::pair<int, int> key(chunk, rows);
+  auto it = blocks.find(key);
   if (it != blocks.end())
     return *it->second;
   auto rb = std::make_unique<RowBlocks>();
-  std::vector<int2> b = row_blocks(h_rowptr.data(), nrows, chunk, kBS);
+  std::vector<int2> b = row_blocks(h_rowptr.data(), nrows, chunk, rows);
   rb->nblocks = static_cast<int64_t>(b.size()) - 1;
   rb->desc.ensure(sizeof(int2) * b.size());
   SPMVX_HIP(hipMemcpy(rb->desc.ptr, b.data(), sizeof(int2) * b.size(), hipMemcpyHostToDevice));
   const RowBlocks& ref = *rb;
-  blocks[chunk] = std::move(rb);
+  blocks[key] = std::move(rb);
   return ref;
 }
 
@@ -193,8 +194,9 @@ int pipe_grid(const void* kernel, int64_t nblocks)
 
 int default_variant(int32_t dtype)
 {
-  // measured best on gfx950 for stencil matrices (bench sweep): FP64 vec 2 x
-  // unroll 2, FP32 vec 4, BF16 vec 4 x unroll 2; SPMVX_SPMV_VARIANT overrides
+  // measured best CSR kernels on gfx950 for the stencil matrices: FP64
+  // 1024-thread workgroups staging 8192 entries (vec 2 x unroll 4), FP32 vec 4,
+  // BF16 vec 4 x unroll 2; SPMVX_SPMV_VARIANT overrides
   static int forced = [] {
     const char* e = std::getenv("SPMVX_SPMV_VARIANT");
     if (!e)
@@ -204,7 +206,7 @@ int default_variant(int32_t dtype)
   }();
   if (forced >= 0)
     return forced;
-  return dtype == SPMVX_F64 ? 2 : (dtype == SPMVX_F32 ? 3 : 4);
+  return dtype == SPMVX_F64 ? 11 : (dtype == SPMVX_F32 ? 3 : 4);
 }
 
 namespace
@@ -322,7 +324,7 @@ __global__ __launch_bounds__(kBS) void sddmm_kernel(const int32_t* __restrict__ 
   const int nr = r1 - r0;
   for (int t = threadIdx.x; t <= nr; t += kBS)
     rp[t] = rowptr[r0 + t];
-  __syncthreads();
+  lds_barrier();
   const int32_t k0 = rp[0], k1 = rp[nr];
   for (int32_t k = k0 + threadIdx.x; k < k1; k += kBS)
   {
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(kBS) void sddmm_kernel(const int32_t* __restrict__ 
 template <typename V, typename X, typename Acc>
 void launch_plain(int var, Pattern& P, const void* vals, const void* x, void* y, hipStream_t s)
 {
-  const RowBlocks& rb = P.blocks_for(variant_block_nnz(var));
+  const RowBlocks& rb = variant_blocks(P, var);
   launch_csr_stream<V, X, Acc, kPlain>(var, rb.nblocks, s, P.rowptr.as<int32_t>(),
                                        P.colidx.as<int32_t>(), static_cast<const V*>(vals),
                                        static_cast<const X*>(x), static_cast<X*>(y),
@@ -397,7 +399,7 @@ std::shared_ptr<Pattern> upload_pattern(int64_t nrows, int64_t ncols, int64_t nn
   if (nnz > 0)
     SPMVX_HIP(hipMemcpy(P->colidx.ptr, colidx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
   P->h_rowptr.assign(rowptr, rowptr + nrows + 1);
-  (void)P->blocks_for(variant_block_nnz(default_variant(SPMVX_F64)));
+  (void)variant_blocks(*P, default_variant(SPMVX_F64));
   return P;
 }
 
@@ -588,7 +590,7 @@ int spmvx_matrix_info(spmvx_matrix A, int32_t* dtype, int64_t* nrows, int64_t* n
     if (nrows) *nrows = A->pat->nrows;
     if (ncols) *ncols = A->pat->ncols;
     if (nnz) *nnz = A->pat->nnz;
-    if (nblocks) *nblocks = A->pat->blocks_for(variant_block_nnz(A->variant)).nblocks;
+    if (nblocks) *nblocks = variant_blocks(*A->pat, A->variant).nblocks;
   });
 }
 
@@ -679,7 +681,7 @@ int spmvx_sddmm(spmvx_matrix A, double alpha, const double* u, const double* v, 
     SPMVX_REQUIRE(A != nullptr, "NULL matrix");
     SPMVX_REQUIRE(A->dtype == SPMVX_F64, "sddmm needs an FP64 matrix");
     Pattern& P = *A->pat;
-    const RowBlocks& rb = P.blocks_for(variant_block_nnz(A->variant));
+    const RowBlocks& rb = P.blocks_for(2048, kBS); // the kernel stages <= kBS row pointers
     if (rb.nblocks == 0 || P.nnz == 0)
       return;
     SPMVX_REQUIRE(u && v && out, "NULL vector");
