@@ -300,6 +300,27 @@ def main():
     log(f"[spmv] step {ms_per_step:.3f} ms (events {step_ms_ev:.3f}), SpMV kernel {spmv_ms:.4f} ms "
         f"= {achieved:.0f} GB/s")
 
+    # ---- the same product with the sliced row-pattern layout (informational)
+    sliced = None
+    try:
+        info = M.sliced_info()
+        M.set_kernel(S.SLICED_VARIANT)
+        y2 = torch.empty_like(y)
+        M.mult(x, out=y2)
+        s_ms = timed_loop(torch, lambda: M.mult(x, out=y2), max(args.steps, 20))
+        Bs = 8 * info["slots"] + 2 * A.nrows + 8 * A.ncols + 8 * A.nrows
+        sliced = {"kernel": "sliced_spmv (kernel variant %d)" % S.SLICED_VARIANT,
+                  "kernel_ms": round(s_ms, 5), "bytes_per_spmv": Bs,
+                  "achieved_GBps": round(Bs / (s_ms * 1e-3) / 1e9, 1),
+                  "csr_equivalent_GBps": round(B / (s_ms * 1e-3) / 1e9, 1),
+                  "speedup_vs_csr": round(spmv_ms / s_ms, 3),
+                  "patterns": info["npatterns"],
+                  "bit_identical_to_csr": bool(torch.equal(y, y2))}
+        log(f"[spmv] sliced layout {s_ms:.4f} ms ({info['npatterns']} patterns)")
+        M.set_kernel(-1)
+    except S.SpmvError as e:
+        sliced = {"unavailable": str(e)}
+
     traffic = None
     tf = os.path.join(ROOT, "profiles", "spmv_27pt_160_traffic.json")
     if os.path.exists(tf):
@@ -327,7 +348,9 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel_ms": round(spmv_ms, 5)},
+                     "traffic": traffic, "kernel_ms": round(spmv_ms, 5),
+                     "kernel": M.kernel_name()},
+        "spmv_sliced": sliced,
     }
 
     if not args.no_cpu:

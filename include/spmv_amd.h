@@ -171,6 +171,9 @@ int spmvx_matrix_arrays(spmvx_matrix A, const int32_t** rowptr_dev,
 #define SPMVX_SLICED_VARIANT 14
 int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant);
 int spmvx_num_kernels(void);
+/* The variant A uses and a short description of its kernel (written into
+ * name[0:name_len), NUL-terminated; name may be NULL). */
+int spmvx_matrix_get_kernel(spmvx_matrix A, int32_t* variant, char* name, int32_t name_len);
 /* Row-pattern analysis of A for the sliced layout (built if needed; no
  * values are copied): number of distinct patterns, longest row and value
  * slots of the layout for A's dtype (nnz plus padding).  SPMVX_ERR_ARG with

@@ -85,6 +85,7 @@ def lib():
         "spmvx_matrix_set_kernel": ([P, i32], ctypes.c_int),
         "spmvx_num_kernels": ([], ctypes.c_int),
         "spmvx_matrix_sliced_info": ([P, P, P, P], ctypes.c_int),
+        "spmvx_matrix_get_kernel": ([P, P, ctypes.c_char_p, i32], ctypes.c_int),
         "spmvx_spmv": ([P, P, P, P], ctypes.c_int),
         "spmvx_spmv_t": ([P, P, P, P], ctypes.c_int),
         "spmvx_cg": ([P, P, P, P, i32, f64, P, P, P], ctypes.c_int),
@@ -361,6 +362,16 @@ class Matrix:
         ``SLICED_VARIANT`` selects the sliced row-pattern layout and raises
         SpmvError with the reason when the matrix does not fit it."""
         _check(lib().spmvx_matrix_set_kernel(self._h, int(variant)))
+
+    def kernel(self):
+        """(variant, description) of the SpMV kernel this matrix uses."""
+        v = ctypes.c_int32()
+        buf = ctypes.create_string_buffer(160)
+        _check(lib().spmvx_matrix_get_kernel(self._h, ctypes.byref(v), buf, len(buf)))
+        return v.value, buf.value.decode()
+
+    def kernel_name(self):
+        return "variant %d: %s" % self.kernel()
 
     def sliced_info(self):
         """Row-pattern analysis for the sliced layout: dict(npatterns, wmax,

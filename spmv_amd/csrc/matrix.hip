@@ -639,6 +639,32 @@ int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant)
 
 int spmvx_num_kernels(void) { return kSlicedVariant + 1; }
 
+int spmvx_matrix_get_kernel(spmvx_matrix A, int32_t* variant, char* name, int32_t name_len)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    const int v = A->variant;
+    if (variant)
+      *variant = v;
+    if (name && name_len > 0)
+    {
+      std::string d;
+      if (v == kSlicedVariant)
+        d = "sliced_spmv (row-pattern slices of 64 rows)";
+      else
+      {
+        const SpmvVariant& k = kVariants[v];
+        d = std::string(k.kind == 0 ? "csr_stream" : "csr_pipe") + " (" + std::to_string(k.bs)
+            + " threads, chunk " + std::to_string(k.chunk) + ", vec " + std::to_string(k.vec)
+            + ", unroll " + std::to_string(k.unroll) + (k.nt ? ", non-temporal" : "") + ")";
+      }
+      const size_t n = std::min(d.size(), static_cast<size_t>(name_len - 1));
+      std::memcpy(name, d.data(), n);
+      name[n] = '\0';
+    }
+  });
+}
+
 int spmvx_matrix_sliced_info(spmvx_matrix A, int32_t* npatterns, int32_t* wmax, int64_t* slots)
 {
   return guarded([&] {

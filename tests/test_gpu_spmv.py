@@ -195,6 +195,7 @@ def test_all_kernel_variants_bitexact(name, A):
         if v == S.SLICED_VARIANT and name != "stencil27_ghost":
             continue  # no fit: covered by test_sliced_layout_rejects
         M.set_kernel(v)
+        assert M.kernel()[0] == v and M.kernel()[1]
         assert np.array_equal(M.mult(x).cpu().numpy(), want), v
         Mb.set_kernel(v)
         yb = Mb.mult(xf).cpu().numpy()
