@@ -160,3 +160,70 @@ def test_refinement_sliced_inner(dtype):
     assert rel <= 1e-10
     assert st2["outer_its"] == st1["outer_its"]
     assert abs(st2["inner_its"] - st1["inner_its"]) <= 3 * st1["outer_its"]
+
+
+# ---------------------------------------------------------------- known answers
+def _golden():
+    import glob
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    for path in sorted(glob.glob(os.path.join(here, "golden", "spmv", "*.json"))):
+        with open(path) as f:
+            yield json.load(f)
+
+
+GOLDEN = list(_golden())
+_GEN = {
+    "demo1d_64": lambda: P.demo1d(64),
+    "poisson5_8x8": lambda: P.stencil(5, 8, 8),
+    "poisson7_6x6x6": lambda: P.stencil(7, 6, 6, 6),
+    "stencil27_5x5x5": lambda: P.stencil(27, 5, 5, 5),
+    "stencil27_5x5x5_ghost": lambda: P.stencil(27, 5, 5, 5, ghost_planes=1),
+}
+
+
+def _unhex(a):
+    return np.array([float.fromhex(v) for v in a])
+
+
+@pytest.mark.parametrize("g", GOLDEN, ids=[g["case"] for g in GOLDEN])
+def test_golden_vectors(g):
+    """SpMV bit-identical to the fixture (tests/golden/spmv), CG within the
+    contract (iterations +-1, rho history 1e-10) of the fixture's history."""
+    A = _GEN[g["case"]]()
+    assert (A.nrows, A.ncols, A.nnz) == (g["nrows"], g["ncols"], g["nnz"])
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    x = P.uniform(g["x_seed"], A.ncols, *g["x_range"])
+    assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), _unhex(g["y"]))
+    if "cg_iterations" in g:
+        b = _unhex(g["b"])
+        xg, k, hist = S.cg(M, _dev(b), 10000, g["cg_rtol"], history=True)
+        ko, histo = g["cg_iterations"], _unhex(g["cg_history"])
+        assert abs(k - ko) <= 1
+        m = min(k, ko) + 1
+        assert np.max(np.abs(np.sqrt(hist[:m] / hist[0]) - np.sqrt(histo[:m] / histo[0]))) <= 1e-10
+        xo = _unhex(g["cg_x"])
+        assert np.linalg.norm(xg.cpu().numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def test_config_cpu_plain_poisson5_512():
+    """The reference's CPU configuration at full size: 5-point Poisson on
+    512 x 512 (262,144 rows, 1,308,672 entries), x uniform[-1,1]; 100
+    products (each bit-identical to the oracle), then CG with b = A x_true,
+    x0 = 0, to 1e-8."""
+    A = P.stencil(5, 512, 512)
+    assert (A.nrows, A.nnz) == (262144, 1308672)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    x = P.uniform(11, A.ncols, -1, 1)
+    want = O.spmv(_ocsr(A), x)
+    xd = _dev(x)
+    y = torch.empty_like(xd)
+    for _ in range(100):
+        M.mult(xd, out=y)
+    assert np.array_equal(y.cpu().numpy(), want)
+    xt = P.uniform(12, A.ncols, -1, 1)
+    b = O.spmv(_ocsr(A), xt)
+    xg, k, xo, ko = check_cg(A, b, 20000, 1e-8)
+    r = O.spmv(_ocsr(A), xg) - b
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b) * 1.01
