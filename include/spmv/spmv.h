@@ -5,3 +5,4 @@
 #include "Matrix.h"
 #include "Vector.h"
 #include "cg.h"
+#include "read_petsc.h"

@@ -51,6 +51,7 @@ enum spmvx_dtype
 typedef struct spmvx_matrix_s* spmvx_matrix;
 typedef struct spmvx_l2gmap_s* spmvx_l2gmap;
 typedef struct spmvx_batch_s* spmvx_batch;
+typedef struct spmvx_petsc_csr_s* spmvx_petsc_csr;
 typedef void* spmvx_stream;
 
 /* Last error message of the calling thread ("" if none). */
@@ -211,6 +212,48 @@ int spmvx_spmv_t(spmvx_matrix A, const void* x_dev, void* y_dev, spmvx_stream st
 int spmvx_cg(spmvx_matrix A, spmvx_l2gmap col_map, const double* b_dev, double* x_dev,
              int32_t max_its, double rtol, int32_t* its, double* hist_host,
              spmvx_stream stream);
+
+/* ===================================================================== */
+/* PETSc binary input (spmv/read_petsc.h:10-31, spmv/read_petsc.cpp)      */
+/* ===================================================================== */
+/*
+ * Files as PETSc writes them with -ksp_view_mat / -ksp_view_rhs binary:
+ * big-endian, 32-bit indices.  Rows (and columns) are split into `nranks`
+ * near-equal contiguous ranges, the first (N mod nranks) one longer
+ * (owner_ranges, read_petsc.cpp:13-30); rank `rank` gets its rows, its
+ * owned columns numbered first and the other referenced columns as ghosts
+ * in ascending global order (read_petsc.cpp:115-133), every row sorted by
+ * local column as Eigen stores it (read_petsc.cpp:152-160).
+ * Errors carry the reference's texts ("Could not open file", "Bad signature
+ * in PETSc Matrix file", "Bad signature in PETSc Vector file"), plus
+ * truncation / range / duplicate-column checks the reference leaves to
+ * assert.
+ */
+/* Host-side parse only (no GPU needed): local CSR, ghosts and, for the
+ * exchange plan, which of this rank's columns every other rank ghosts. */
+int spmvx_petsc_csr_read(const char* path, int32_t nranks, int32_t rank, spmvx_petsc_csr* out);
+int spmvx_petsc_csr_destroy(spmvx_petsc_csr h);
+int spmvx_petsc_csr_info(spmvx_petsc_csr h, int64_t* nrows_global, int64_t* ncols_global,
+                         int64_t* nrows_local, int64_t* ncols_local, int64_t* nghosts,
+                         int64_t* nnz_local);
+/* Host pointers owned by h (valid until spmvx_petsc_csr_destroy). */
+int spmvx_petsc_csr_arrays(spmvx_petsc_csr h, const int32_t** rowptr, const int32_t** colidx,
+                           const double** values, const int64_t** ghosts);
+/* For the exchange plan: remote_ghosts[remote_ptr[q]:remote_ptr[q+1]) are the
+ * columns of this rank that rank q ghosts (ascending; empty for q = rank),
+ * the remote_* arguments of spmvx_l2gmap_create. */
+int spmvx_petsc_csr_remote(spmvx_petsc_csr h, const int64_t** remote_ptr,
+                           const int64_t** remote_ghosts);
+/* Replaces spmv::read_petsc_binary (read_petsc.cpp:32-191): the matrix (values
+ * stored as dtype), its column map (the exchange plan of this rank) and its
+ * row map (no ghosts; may be NULL). */
+int spmvx_petsc_read_matrix(const char* path, int32_t nranks, int32_t rank, int32_t dtype,
+                            spmvx_matrix* A, spmvx_l2gmap* col_map, spmvx_l2gmap* row_map);
+/* Replaces spmv::read_petsc_binary_vector (read_petsc.cpp:193-260): this
+ * rank's part into out_host[0:n_local) (host memory, capacity entries).
+ * out_host = NULL only reports n_local and n_global. */
+int spmvx_petsc_read_vector(const char* path, int32_t nranks, int32_t rank, double* out_host,
+                            int64_t capacity, int64_t* n_local, int64_t* n_global);
 
 /* ===================================================================== */
 /* Extension 1: mixed-precision iterative refinement (no reference        */

@@ -24,6 +24,7 @@ import os
 import numpy as np
 
 __all__ = ["L2GMap", "Matrix", "Batch", "cg", "cg_f32", "ir_cg", "sddmm", "copy_f64", "SpmvError",
+           "read_petsc_csr", "read_petsc_matrix", "read_petsc_vector",
            "F64", "F32", "BF16", "SLICED_VARIANT", "lib", "library_path"]
 
 F64, F32, BF16 = 0, 1, 2
@@ -93,6 +94,13 @@ def lib():
         "spmvx_ir_cg": ([P, P, P, P, i32, f64, i32, f64, i32, P, P], ctypes.c_int),
         "spmvx_sddmm": ([P, f64, P, P, P, P], ctypes.c_int),
         "spmvx_copy_f64": ([P, P, i64, P], ctypes.c_int),
+        "spmvx_petsc_csr_read": ([ctypes.c_char_p, i32, i32, P], ctypes.c_int),
+        "spmvx_petsc_csr_destroy": ([P], ctypes.c_int),
+        "spmvx_petsc_csr_info": ([P, P, P, P, P, P, P], ctypes.c_int),
+        "spmvx_petsc_csr_arrays": ([P, P, P, P, P], ctypes.c_int),
+        "spmvx_petsc_csr_remote": ([P, P, P], ctypes.c_int),
+        "spmvx_petsc_read_matrix": ([ctypes.c_char_p, i32, i32, i32, P, P, P], ctypes.c_int),
+        "spmvx_petsc_read_vector": ([ctypes.c_char_p, i32, i32, P, i64, P, P], ctypes.c_int),
         "spmvx_batch_create": ([i32, P, P, P, P, i32, P], ctypes.c_int),
         "spmvx_batch_destroy": ([P], ctypes.c_int),
         "spmvx_batch_set_config": ([P, i32, i32, i32], ctypes.c_int),
@@ -524,3 +532,70 @@ class Batch:
         if iterations:
             return x, its.reshape(self.nsys, self.nrhs)
         return x
+
+
+# ---------------------------------------------------------------- PETSc input
+def _host_array(ptr, n, ctype, dtype):
+    if n == 0 or not ptr.value:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), (n,)).astype(dtype)
+
+
+def read_petsc_csr(path, nranks=1, rank=0):
+    """Host-side parse of a PETSc binary matrix file for rank `rank` of an
+    `nranks`-way row split (spmv/read_petsc.cpp:32-191): dict with the local
+    CSR (rowptr, colidx in local column numbering, values), ghosts (global,
+    ascending), sizes, and remote (for each rank q, the columns of this rank
+    that q ghosts)."""
+    h = ctypes.c_void_p()
+    _check(lib().spmvx_petsc_csr_read(os.fsencode(path), int(nranks), int(rank), ctypes.byref(h)))
+    try:
+        v = [ctypes.c_int64() for _ in range(6)]
+        _check(lib().spmvx_petsc_csr_info(h, *[ctypes.byref(x) for x in v]))
+        nrows, ncols, nrl, ncl, ngh, nnz = [x.value for x in v]
+        p = [ctypes.c_void_p() for _ in range(4)]
+        _check(lib().spmvx_petsc_csr_arrays(h, *[ctypes.byref(x) for x in p]))
+        rp = [ctypes.c_void_p() for _ in range(2)]
+        _check(lib().spmvx_petsc_csr_remote(h, *[ctypes.byref(x) for x in rp]))
+        rptr = _host_array(rp[0], nranks + 1, ctypes.c_int64, np.int64)
+        rg = _host_array(rp[1], int(rptr[-1]), ctypes.c_int64, np.int64)
+        return {"nrows": nrows, "ncols": ncols, "nrows_local": nrl, "ncols_local": ncl,
+                "rowptr": _host_array(p[0], nrl + 1, ctypes.c_int32, np.int64),
+                "colidx": _host_array(p[1], nnz, ctypes.c_int32, np.int64),
+                "values": _host_array(p[2], nnz, ctypes.c_double, np.float64),
+                "ghosts": _host_array(p[3], ngh, ctypes.c_int64, np.int64),
+                "remote": [rg[rptr[q]:rptr[q + 1]] for q in range(nranks)]}
+    finally:
+        lib().spmvx_petsc_csr_destroy(h)
+
+
+def read_petsc_matrix(path, nranks=1, rank=0, dtype="f64"):
+    """spmv::read_petsc_binary (read_petsc.cpp:32-191): the Matrix of rank
+    `rank` with its column map (the exchange plan of an `nranks`-way row
+    split) and row map attached."""
+    c = read_petsc_csr(path, nranks, rank)
+    cr = _owner_ranges(nranks, c["ncols"])
+    rr = _owner_ranges(nranks, c["nrows"])
+    col_map = L2GMap(c["ncols_local"], c["ghosts"], local_sizes=np.diff(cr), rank=rank,
+                     remote_ghosts=c["remote"])
+    row_map = L2GMap(c["nrows_local"], (), local_sizes=np.diff(rr), rank=rank)
+    return Matrix(c["rowptr"], c["colidx"], c["values"],
+                  ncols=c["ncols_local"] + len(c["ghosts"]), col_map=col_map, row_map=row_map,
+                  dtype=dtype)
+
+
+def read_petsc_vector(path, nranks=1, rank=0):
+    """spmv::read_petsc_binary_vector (read_petsc.cpp:193-260): this rank's
+    part as a host array."""
+    nl, ng = ctypes.c_int64(), ctypes.c_int64()
+    _check(lib().spmvx_petsc_read_vector(os.fsencode(path), int(nranks), int(rank), None, 0,
+                                         ctypes.byref(nl), ctypes.byref(ng)))
+    out = np.zeros(max(nl.value, 1), np.float64)
+    _check(lib().spmvx_petsc_read_vector(os.fsencode(path), int(nranks), int(rank),
+                                         _np_ptr(out), nl.value, None, None))
+    return out[:nl.value]
+
+
+def _owner_ranges(size, n):
+    q, r = divmod(int(n), int(size))
+    return np.array([k * (q + 1) if k < r else k * q + r for k in range(size + 1)], np.int64)

@@ -3,6 +3,7 @@
 // message, which for the reference's checks is the reference's text.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -311,6 +312,82 @@ std::tuple<Vector<double>, int> cg(MPI_Comm comm, const Matrix<double>& A,
   check(spmvx_cg(A.handle(), col ? col->handle() : nullptr, b.data(), x.data(), max_its, rtol,
                  &its, nullptr, nullptr));
   return std::make_tuple(std::move(x), static_cast<int>(its));
+}
+
+// ---------------------------------------------------------------- PETSc input
+namespace
+{
+std::vector<std::int64_t> split(int size, std::int64_t N) // read_petsc.cpp:13-30
+{
+  const std::int64_t n = N / size, r = N % size;
+  std::vector<std::int64_t> sizes(size);
+  for (int q = 0; q < size; ++q)
+    sizes[q] = n + (q < r ? 1 : 0);
+  return sizes;
+}
+} // namespace
+
+Matrix<double> read_petsc_binary(MPI_Comm comm, std::string filename, int nranks, int rank)
+{
+  if (comm_size(comm) != 1)
+    throw std::runtime_error("spmv::read_petsc_binary: one process drives one rank here");
+  spmvx_petsc_csr h = nullptr;
+  check(spmvx_petsc_csr_read(filename.c_str(), nranks, rank, &h));
+  std::unique_ptr<spmvx_petsc_csr_s, int (*)(spmvx_petsc_csr)> hold(h, spmvx_petsc_csr_destroy);
+  std::int64_t nr = 0, nc = 0, nrl = 0, ncl = 0, ng = 0, nnz = 0;
+  check(spmvx_petsc_csr_info(h, &nr, &nc, &nrl, &ncl, &ng, &nnz));
+  if (rank == 0)
+    std::printf("Read file: %s: %lldx%lld = %lld\n", filename.c_str(),
+                static_cast<long long>(nr), static_cast<long long>(nc),
+                static_cast<long long>(nnz)); // read_petsc.cpp:72-74
+  const std::int32_t *rp = nullptr, *ci = nullptr;
+  const double* val = nullptr;
+  const std::int64_t* gh = nullptr;
+  check(spmvx_petsc_csr_arrays(h, &rp, &ci, &val, &gh));
+  const std::int64_t *rptr = nullptr, *rgh = nullptr;
+  check(spmvx_petsc_csr_remote(h, &rptr, &rgh));
+  CSR<double> A;
+  A.rows = nrl;
+  A.cols = ncl + ng;
+  A.outer.assign(rp, rp + nrl + 1);
+  A.inner.assign(ci, ci + nnz);
+  A.values.assign(val, val + nnz);
+  Decomposition dc, dr;
+  dc.local_sizes = split(nranks, nc);
+  dc.rank = rank;
+  dc.remote_ghosts.resize(nranks);
+  for (int q = 0; q < nranks; ++q)
+    dc.remote_ghosts[q].assign(rgh + rptr[q], rgh + rptr[q + 1]);
+  dr.local_sizes = split(nranks, nr);
+  dr.rank = rank;
+  auto col_map = std::make_shared<L2GMap>(comm, ncl, std::vector<std::int64_t>(gh, gh + ng), dc);
+  auto row_map = std::make_shared<L2GMap>(comm, nrl, std::vector<std::int64_t>{}, dr);
+  return Matrix<double>(A, col_map, row_map);
+}
+
+Matrix<double> read_petsc_binary(MPI_Comm comm, std::string filename)
+{
+  return read_petsc_binary(comm, std::move(filename), 1, 0);
+}
+
+std::vector<double> read_petsc_binary_vector(MPI_Comm comm, std::string filename, int nranks,
+                                             int rank)
+{
+  if (comm_size(comm) != 1)
+    throw std::runtime_error("spmv::read_petsc_binary_vector: one process drives one rank here");
+  std::int64_t nl = 0, ng = 0;
+  check(spmvx_petsc_read_vector(filename.c_str(), nranks, rank, nullptr, 0, &nl, &ng));
+  if (rank == 0)
+    std::printf("Read vector file: %s: %lld\n", filename.c_str(),
+                static_cast<long long>(ng)); // read_petsc.cpp:228-229
+  std::vector<double> v(static_cast<std::size_t>(nl));
+  check(spmvx_petsc_read_vector(filename.c_str(), nranks, rank, v.data(), nl, nullptr, nullptr));
+  return v;
+}
+
+std::vector<double> read_petsc_binary_vector(MPI_Comm comm, std::string filename)
+{
+  return read_petsc_binary_vector(comm, std::move(filename), 1, 0);
 }
 
 } // namespace spmv

@@ -1,14 +1,19 @@
-// CG demo through the C++ API (the structure of demos/cgmain.cpp:16-108):
-// 2D 5-point Poisson on an n x n grid (Dirichlet), x_true uniform[-1,1] from
-// a splitmix64 stream, b = A x_true, x = spmv::cg(A, b), then the true
-// residual ||A x - b|| (cgmain.cpp:69-73).
+// CG demo through the C++ API (the structure of demos/cgmain.cpp:16-108).
+//   cgmain petsc <name>
+//     as the reference: A from petsc_mat<name>.dat, b from petsc_vec<name>.dat
+//     (PETSc binary files), x = spmv::cg(A, b, 10000, 1e-10), then the true
+//     residual ||A x - b|| and ||x|| (cgmain.cpp:36-81).
 //   cgmain [n=512] [rtol=1e-8] [max_its=10000] [seed=1]
+//     generated input: 2D 5-point Poisson on an n x n grid (Dirichlet),
+//     x_true uniform[-1,1] from a splitmix64 stream, b = A x_true.
 #include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <string>
+#include <vector>
 
 #include <spmv/spmv.h>
 
@@ -21,9 +26,40 @@ static double uniform_pm1(std::uint64_t seed, std::uint64_t i)
   return -1.0 + 2.0 * (static_cast<double>(z >> 11) * 0x1.0p-53);
 }
 
+static int petsc_main(const std::string& name)
+{
+  auto t0 = std::chrono::steady_clock::now();
+  auto A = spmv::read_petsc_binary(MPI_COMM_WORLD, "petsc_mat" + name + ".dat");
+  std::shared_ptr<const spmv::L2GMap> l2g = A.col_map();
+  spmv::Vector<double> b(spmv::read_petsc_binary_vector(MPI_COMM_WORLD, "petsc_vec" + name + ".dat"));
+  std::printf("Global vec size = %lld\n", static_cast<long long>(l2g->global_size()));
+  const double read = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  t0 = std::chrono::steady_clock::now();
+  auto [x, num_its] = spmv::cg(MPI_COMM_WORLD, A, b, 10000, 1e-10);
+  const double solve = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  l2g->update(x.data());
+  std::vector<double> ax = (A * x).to_host(), bh = b.to_host(), xh = x.to_host();
+  double rn = 0, xn = 0;
+  for (std::size_t i = 0; i < bh.size(); ++i)
+  {
+    rn += (ax[i] - bh[i]) * (ax[i] - bh[i]);
+    xn += xh[i] * xh[i];
+  }
+  std::printf("r.norm = %.17g\n", std::sqrt(rn));
+  std::printf("x.norm = %.17g in %d iterations\n", std::sqrt(xn), num_its);
+  std::printf("[0.ReadPetsc] %.6f\n[1.Solve] %.6f\n", read, solve);
+  return 0;
+}
+
 int main(int argc, char** argv)
 {
   MPI_Init(&argc, &argv);
+  if (argc > 2 && std::string(argv[1]) == "petsc")
+  {
+    const int rc = petsc_main(argv[2]);
+    MPI_Finalize();
+    return rc;
+  }
   const std::int64_t n = argc > 1 ? std::atoll(argv[1]) : 512;
   const double rtol = argc > 2 ? std::atof(argv[2]) : 1e-8;
   const int max_its = argc > 3 ? std::atoi(argv[3]) : 10000;
