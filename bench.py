@@ -161,11 +161,23 @@ def run_batch_section(torch, S, P, nsys=256, nrhs=4, rtol=1e-6, seed=17):
         for j in range(nrhs):
             r = bh[lo:hi, j].astype(np.float64) - O.spmv(oc, xh[lo:hi, j].astype(np.float64))
             worst = max(worst, float(np.linalg.norm(r) / np.linalg.norm(bh[lo:hi, j])))
+    # algorithmic bytes: every right-hand side's iterations x (FP32 SpMV bytes
+    # + 9 vector passes), the kernel reading each matrix once for all its RHS
+    n_s = np.array([A.nrows for A in systems], np.float64)
+    nnz_s = np.array([A.nnz for A in systems], np.float64)
+    per_it = nnz_s * 8 + 4 * (n_s + 1) + 8 * n_s + 9 * 4 * n_s
+    alg = float(np.sum(its.astype(np.float64) * per_it[:, None]))
+    # what one iteration of a system moves when its RHS share the matrix
+    shared = float(np.sum(its.max(axis=1) * (nnz_s * 8 + 4 * (n_s + 1)))
+                   + np.sum(its.astype(np.float64) * (8 * n_s + 9 * 4 * n_s)[:, None]))
     out = {"workload": f"{nsys} random SPD systems x {nrhs} RHS, {B.nrows} rows, {B.nnz} nnz, FP32, rtol {rtol}",
            "seconds": ts, "systems_per_s": nsys / ts, "rhs_solves_per_s": nsys * nrhs / ts,
+           "algorithmic_bytes": alg, "algorithmic_GBps": alg / ts / 1e9,
+           "bytes_matrix_shared_by_rhs": shared, "shared_GBps": shared / ts / 1e9,
            "iterations_max": int(its.max()), "iterations_mean": float(its.mean()),
            "config": B.config(), "tuned": best, "worst_true_rel_residual_sampled": worst}
-    log(f"[batch] {ts:.3f}s its max {its.max()} mean {its.mean():.1f} config {B.config()}")
+    log(f"[batch] {ts:.3f}s its max {its.max()} mean {its.mean():.1f} config {B.config()} "
+        f"{alg / ts / 1e9:.0f} GB/s algorithmic, {shared / ts / 1e9:.0f} GB/s with shared matrix reads")
     del B
     return out
 

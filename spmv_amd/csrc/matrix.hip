@@ -470,8 +470,17 @@ SlicedView sliced_view(spmvx_matrix_s* A)
     SPMVX_HIP(hipDeviceSynchronize());
     A->svalues_ready = true;
   }
+  // slices per wavefront: SPMVX_SLICED_RPL (1, 2 or 4) overrides the default
+  static const int forced = [] {
+    const char* e = std::getenv("SPMVX_SLICED_RPL");
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 1 || v == 2 || v == 4) ? v : 0;
+  }();
+  // measured on the 27- and 7-point stencils: FP64 one slice per wavefront,
+  // FP32 four, BF16 two
+  const int bs = forced ? forced : (A->dtype == SPMVX_F64 ? 1 : (A->dtype == SPMVX_F32 ? 4 : 2));
   return SlicedView{R->pid.as<uint16_t>(), L.sdesc.as<int2>(), L.nslices, P.nrows, P.ncols,
-                    R->dict.as<int32_t>(),  R->dict_n,          R->wmax};
+                    R->dict.as<int32_t>(),  R->dict_n,          R->wmax,   bs};
 }
 
 namespace
