@@ -214,10 +214,13 @@ def run_adjoint_section(torch, S, P, fd_entries, n=1024, rtol=1e-13, seed=1):
     g = torch.empty(op.A.nnz, dtype=torch.float64, device=dev)
     sd_ms = timed_loop(torch, lambda: S.sddmm(op.A, lam, x.detach(), alpha=-1.0, out=g), 20)
     sd_bytes = op.A.nnz * (4 + 8) + 4 * (op.A.nrows + 1) + 2 * 8 * op.A.nrows
+    its = getattr(op.A, "cg_iterations", {})
     out = {"workload": f"anisotropic 5-point diffusion {n}^2, {op.A.nnz} nnz, {ne} edge conductivities",
            "forward_seconds": tf, "backward_seconds": tb, "rtol": rtol, "loss": loss.item(),
+           "forward_cg_iterations": its.get("forward"), "adjoint_cg_iterations": its.get("adjoint"),
            "sddmm_ms": sd_ms, "sddmm_GBps": sd_bytes / sd_ms / 1e6}
-    log(f"[adjoint] forward {tf:.3f}s backward {tb:.3f}s sddmm {sd_ms:.3f} ms")
+    log(f"[adjoint] forward {tf:.3f}s ({its.get('forward')} its) backward {tb:.3f}s "
+        f"({its.get('adjoint')} its) sddmm {sd_ms:.3f} ms")
     if fd_entries > 0:
         entries = (P.rand_u64(77, fd_entries) % np.uint64(ne)).astype(np.int64)
         t = time.perf_counter()

@@ -38,6 +38,7 @@ class SparseSolve(torch.autograd.Function):
         x, its = cg(A, b.detach().contiguous(), max_its, rtol)
         ctx.A, ctx.max_its, ctx.rtol = A, max_its, rtol
         ctx.its = [its]
+        A.cg_iterations = {"forward": its}  # reporting only
         ctx.save_for_backward(x)
         return x
 
@@ -52,6 +53,7 @@ class SparseSolve(torch.autograd.Function):
         else:
             lam, its = cg(A, grad_x.contiguous(), ctx.max_its, ctx.rtol)
             ctx.its.append(its)
+            getattr(A, "cg_iterations", {})["adjoint"] = its
         grad_values = sddmm(A, lam, x, alpha=-1.0) if ctx.needs_input_grad[0] else None
         grad_b = lam if ctx.needs_input_grad[1] else None
         return grad_values, grad_b, None, None, None

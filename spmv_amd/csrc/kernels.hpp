@@ -78,14 +78,24 @@ template <typename Acc>
 __device__ __forceinline__ Acc finalize_sum(const Acc* __restrict__ partials, int64_t n,
                                             Acc* red)
 {
+  // 16 independent loads per thread and step (latency-bound on tens of
+  // thousands of partials), four accumulators, fixed order
   Acc s0 = Acc(0), s1 = Acc(0), s2 = Acc(0), s3 = Acc(0);
   int64_t i = threadIdx.x;
-  for (; i + 3 * kFinBS < n; i += 4 * kFinBS)
+  for (; i + 15 * kFinBS < n; i += 16 * kFinBS)
   {
-    s0 += partials[i];
-    s1 += partials[i + kFinBS];
-    s2 += partials[i + 2 * kFinBS];
-    s3 += partials[i + 3 * kFinBS];
+    Acc v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      v[u] = partials[i + u * kFinBS];
+#pragma unroll
+    for (int u = 0; u < 16; u += 4)
+    {
+      s0 += v[u];
+      s1 += v[u + 1];
+      s2 += v[u + 2];
+      s3 += v[u + 3];
+    }
   }
   for (; i < n; i += kFinBS)
     s0 += partials[i];
