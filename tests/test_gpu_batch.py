@@ -40,6 +40,11 @@ def test_batch_matches_oracle(nrhs, rpm):
             r = b[lo:hi, j].astype(np.float64) - O.spmv(oc, xh[lo:hi, j].astype(np.float64))
             assert np.linalg.norm(r) <= 5e-6 * bn
             assert np.linalg.norm(xh[lo:hi, j] - xo) <= 1e-4 * np.linalg.norm(xo)
+            # the recursive residual stops below rtol; the true residual of an
+            # FP32 CG sits near 1e-6 (FP32 rounding of x and r), as the
+            # oracle's own FP32 CG shows: no worse than it, or below rtol
+            ro = b[lo:hi, j].astype(np.float64) - O.spmv(oc, xo.astype(np.float64))
+            assert np.linalg.norm(r) <= max(1.25 * np.linalg.norm(ro), 1e-6 * bn)
 
 
 def true_rel_residuals(systems, B, b, x):
