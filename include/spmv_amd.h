@@ -107,7 +107,9 @@ int spmvx_l2gmap_global_to_local(spmvx_l2gmap map, int64_t global_index, int32_t
  * Single-process transport.  What the neighbours would send into this rank's
  * ghost region comes from a host "halo" buffer of num_ghosts values of
  * `dtype` (F64 or F32), copied into library-owned pinned memory; NULL means
- * zeros (the default).
+ * zeros (the default).  The unpack kernel of every later update reads that
+ * memory directly, so call this only when no update is in flight
+ * (synchronise the stream first).
  */
 int spmvx_l2gmap_set_halo(spmvx_l2gmap map, int32_t dtype, const void* halo_host);
 

@@ -76,7 +76,6 @@ void l2gmap_prepare_device(spmvx_l2gmap_s* m)
   if (nidx)
     SPMVX_HIP(hipMemcpy(m->d_indexbuf.ptr, m->indexbuf.data(), 4 * nidx, hipMemcpyHostToDevice));
   m->d_send.ensure(8 * std::max<size_t>(nidx, 1));
-  m->d_inbox.ensure(8 * std::max<size_t>(ng, 1)); // zeroed
   const size_t nt = m->rev_target.size();
   m->d_rev_target.ensure(4 * std::max<size_t>(nt, 1));
   m->d_rev_ptr.ensure(4 * (nt + 1));
@@ -98,13 +97,15 @@ void l2gmap_update_async(spmvx_l2gmap_s* m, int32_t dtype, void* vec, hipStream_
   if (nidx == 0 && ng == 0)
     return;
   const size_t es = vec_elem(dtype);
-  // receive: the halo (what the neighbours send) lands in the device inbox
+  // receive: the unpack kernel reads the halo (what the neighbours send)
+  // straight from its pinned host buffer.  No separate host-to-device copy:
+  // it would be one more operation per update, and HIP enqueues a pinned
+  // copy on a stream that waits for an event by blocking the host.
   const void* inbox = nullptr;
   if (ng > 0 && m->halo_dtype >= 0)
   {
     SPMVX_REQUIRE(m->halo_dtype == dtype, "halo dtype differs from the vector dtype");
-    SPMVX_HIP(hipMemcpyAsync(m->d_inbox.ptr, m->h_halo.ptr, es * ng, hipMemcpyHostToDevice, s));
-    inbox = m->d_inbox.ptr;
+    inbox = m->d_halo_view;
   }
   const int g = grid_for(std::max(nidx, ng));
   if (dtype == SPMVX_F64)
@@ -323,6 +324,7 @@ int spmvx_l2gmap_set_halo(spmvx_l2gmap m, int32_t dtype, const void* halo_host)
     m->h_halo.ensure(es * std::max<size_t>(ng, 1));
     if (ng)
       std::memcpy(m->h_halo.ptr, halo_host, es * ng);
+    SPMVX_HIP(hipHostGetDevicePointer(&m->d_halo_view, m->h_halo.ptr, 0));
     m->halo_dtype = dtype;
   });
 }
