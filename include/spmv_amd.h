@@ -18,6 +18,12 @@
  *   - Vector precision follows the matrix value type: SPMVX_F64 matrices
  *     take double vectors; SPMVX_F32 and SPMVX_BF16 matrices take float
  *     vectors (BF16 values are widened to FP32, FP32 accumulation).
+ *   - Complex matrices (SPMVX_C64 / SPMVX_C32, the reference's
+ *     Matrix<std::complex<T>>, Matrix.cpp:441-444) take complex vectors of
+ *     the same precision, (re, im) interleaved; host value arrays are then
+ *     2*nnz doubles.  They support the products, transposed products, value
+ *     updates and the L2GMap operations (kernel variants 0-4); CG, refinement,
+ *     SDDMM, the sliced layout and create_like are for real values.
  *   - Handles are not thread safe; calls on one handle must be serialised.
  */
 #ifndef SPMV_AMD_H
@@ -45,7 +51,9 @@ enum spmvx_dtype
 {
   SPMVX_F64 = 0,
   SPMVX_F32 = 1,
-  SPMVX_BF16 = 2 /* matrix values only */
+  SPMVX_BF16 = 2, /* matrix values only */
+  SPMVX_C64 = 3,  /* complex double: (re, im) pairs of doubles */
+  SPMVX_C32 = 4   /* complex float: (re, im) pairs of floats */
 };
 
 typedef struct spmvx_matrix_s* spmvx_matrix;

@@ -58,6 +58,9 @@ def lib():
         L.or_spmv_f64_omp.argtypes = [_i64, _i32p, _i32p, _f64p, _f64p, _f64p, ctypes.c_int]
         L.or_spmv_t_f64.argtypes = [_i64, _i64, _i32p, _i32p, _f64p, _f64p, _f64p]
         L.or_spmv_f32.argtypes = [_i64, _i32p, _i32p, _f32p, _f32p, _f32p]
+        L.or_spmv_c128.argtypes = [_i64, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.or_spmv_c64.argtypes = [_i64, _i32p, _i32p, _f32p, _f32p, _f32p]
+        L.or_spmv_t_c128.argtypes = [_i64, _i64, _i32p, _i32p, _f64p, _f64p, _f64p]
         L.or_l2g_update_f64.argtypes = [_i64, _i64, _i64, _vp, _vp, _vp, _vp]
         L.or_l2g_reverse_update_f64.argtypes = [_i64, _i64, _i64, _vp, _vp, _vp, _vp]
         L.or_cg_f64.argtypes = [_i64, _i64, _i32p, _i32p, _f64p, _f64p, ctypes.c_int,
@@ -183,6 +186,30 @@ def spmv_f32(A, val32, x):
     lib().or_spmv_f32(A.nrows, A.rowptr, A.colidx, np.ascontiguousarray(val32, np.float32),
                       np.ascontiguousarray(x, np.float32), y)
     return y
+
+
+def spmv_complex(A, values, x):
+    """Complex product in the reference's row order; values and x complex128
+    (or complex64 for single precision)."""
+    if np.asarray(values).dtype == np.complex64:
+        val = np.ascontiguousarray(values, np.complex64).view(np.float32)
+        xx = np.ascontiguousarray(x, np.complex64).view(np.float32)
+        y = np.empty(2 * A.nrows, np.float32)
+        lib().or_spmv_c64(A.nrows, A.rowptr, A.colidx, val, xx, y)
+        return y.view(np.complex64)
+    val = np.ascontiguousarray(values, np.complex128).view(np.float64)
+    xx = np.ascontiguousarray(x, np.complex128).view(np.float64)
+    y = np.empty(2 * A.nrows, np.float64)
+    lib().or_spmv_c128(A.nrows, A.rowptr, A.colidx, val, xx, y)
+    return y.view(np.complex128)
+
+
+def spmv_t_complex(A, values, x):
+    val = np.ascontiguousarray(values, np.complex128).view(np.float64)
+    xx = np.ascontiguousarray(x, np.complex128).view(np.float64)
+    y = np.empty(2 * A.ncols, np.float64)
+    lib().or_spmv_t_c128(A.nrows, A.ncols, A.rowptr, A.colidx, val, xx, y)
+    return y.view(np.complex128)
 
 
 def l2g_update(local_size, num_ghosts, indexbuf, v, halo):

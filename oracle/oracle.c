@@ -91,6 +91,71 @@ void or_spmv_f64_omp(int64_t nrows, const int32_t* rowptr,
   (void)nthreads;
 }
 
+/* Complex products (Matrix<std::complex<T>>, spmv/Matrix.cpp:441-444):
+ * the same row order, std::complex's product without FMA
+ * (re = ar xr - ai xi, im = ar xi + ai xr, each product and sum rounded).
+ * Values and vectors are (re, im) interleaved. */
+void or_spmv_c128(int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
+                  const double* val, const double* x, double* y)
+{
+  for (int64_t i = 0; i < nrows; ++i)
+  {
+    double tr = 0.0, ti = 0.0;
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+    {
+      const double ar = val[2 * (int64_t)k], ai = val[2 * (int64_t)k + 1];
+      const double xr = x[2 * (int64_t)colidx[k]], xi = x[2 * (int64_t)colidx[k] + 1];
+      const double pr = ar * xr - ai * xi;
+      const double pi = ar * xi + ai * xr;
+      tr += pr;
+      ti += pi;
+    }
+    y[2 * i] = tr;
+    y[2 * i + 1] = ti;
+  }
+}
+
+void or_spmv_c64(int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
+                 const float* val, const float* x, float* y)
+{
+  for (int64_t i = 0; i < nrows; ++i)
+  {
+    float tr = 0.0f, ti = 0.0f;
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+    {
+      const float ar = val[2 * (int64_t)k], ai = val[2 * (int64_t)k + 1];
+      const float xr = x[2 * (int64_t)colidx[k]], xi = x[2 * (int64_t)colidx[k] + 1];
+      const float pr = ar * xr - ai * xi;
+      const float pi = ar * xi + ai * xr;
+      tr += pr;
+      ti += pi;
+    }
+    y[2 * i] = tr;
+    y[2 * i + 1] = ti;
+  }
+}
+
+/* Complex transpose product (not conjugated): y[col_k] += a_k x_i, rows in
+ * ascending order, as or_spmv_t_f64. */
+void or_spmv_t_c128(int64_t nrows, int64_t ncols, const int32_t* rowptr,
+                    const int32_t* colidx, const double* val, const double* x, double* y)
+{
+  for (int64_t j = 0; j < 2 * ncols; ++j)
+    y[j] = 0.0;
+  for (int64_t i = 0; i < nrows; ++i)
+  {
+    const double xr = x[2 * i], xi = x[2 * i + 1];
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+    {
+      const double ar = val[2 * (int64_t)k], ai = val[2 * (int64_t)k + 1];
+      const double pr = ar * xr - ai * xi;
+      const double pi = ar * xi + ai * xr;
+      y[2 * (int64_t)colidx[k]] += pr;
+      y[2 * (int64_t)colidx[k] + 1] += pi;
+    }
+  }
+}
+
 /* Transpose product y[ncols] = A^T x[nrows]: spmv/Matrix.cpp:206-211.
  * Eigen evaluates `_matA.transpose() * b` by scattering row i's entries:
  * y[col_k] += a_k * x_i, rows in ascending order. */

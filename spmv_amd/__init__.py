@@ -25,14 +25,27 @@ import numpy as np
 
 __all__ = ["L2GMap", "Matrix", "Batch", "cg", "cg_f32", "ir_cg", "sddmm", "copy_f64", "SpmvError",
            "read_petsc_csr", "read_petsc_matrix", "read_petsc_vector",
-           "F64", "F32", "BF16", "SLICED_VARIANT", "lib", "library_path"]
+           "F64", "F32", "BF16", "C64", "C32", "SLICED_VARIANT", "lib", "library_path"]
 
-F64, F32, BF16 = 0, 1, 2
+F64, F32, BF16, C64, C32 = 0, 1, 2, 3, 4
 # kernel variant of the sliced row-pattern layout (SPMVX_SLICED_VARIANT)
 SLICED_VARIANT = 14
 _DTYPES = {"f64": F64, "float64": F64, "double": F64, F64: F64,
            "f32": F32, "float32": F32, "float": F32, F32: F32,
-           "bf16": BF16, "bfloat16": BF16, BF16: BF16}
+           "bf16": BF16, "bfloat16": BF16, BF16: BF16,
+           "c64": C64, "complex128": C64, C64: C64,
+           "c32": C32, "complex64": C32, C32: C32}
+# numpy element type of each vector dtype
+_NP_VEC = {F64: np.float64, F32: np.float32, C64: np.complex128, C32: np.complex64}
+
+
+def _vec_dtype_of(t):
+    """spmvx dtype of a torch vector."""
+    import torch
+    m = {torch.float64: F64, torch.float32: F32, torch.complex128: C64, torch.complex64: C32}
+    if t.dtype not in m:
+        raise SpmvError(f"unsupported vector dtype {t.dtype}")
+    return m[t.dtype]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(_HERE, "libspmv_amd.so")
@@ -143,7 +156,7 @@ def _dptr(t):
 
 def _vec_torch_dtype(dtype):
     import torch
-    return torch.float64 if dtype == F64 else torch.float32
+    return {F64: torch.float64, C64: torch.complex128, C32: torch.complex64}.get(dtype, torch.float32)
 
 
 # =========================================================================
@@ -240,8 +253,7 @@ class L2GMap:
         if halo is None:
             _check(lib().spmvx_l2gmap_set_halo(self._h, F64, None))
             return
-        npdt = np.float64 if dtype == F64 else np.float32
-        halo = np.ascontiguousarray(halo, npdt)
+        halo = np.ascontiguousarray(halo, _NP_VEC[_DTYPES[dtype]])
         if len(halo) != self.num_ghosts():
             raise SpmvError("halo length differs from num_ghosts")
         _check(lib().spmvx_l2gmap_set_halo(self._h, dtype, _np_ptr(halo)))
@@ -249,24 +261,22 @@ class L2GMap:
     def update(self, vec, send_out=False, stream=None):
         """L2GMap::update on a device vector (in place).  Returns the packed
         send buffer (numpy) when send_out is True."""
-        import torch
-        dt = F64 if vec.dtype == torch.float64 else F32
+        dt = _vec_dtype_of(vec)
         if vec.numel() != self.local_size() + self.num_ghosts():
             raise SpmvError("vector length differs from local_size + num_ghosts")
         buf = None
         if send_out:
-            buf = np.zeros(max(self.num_indices(), 1), np.float64 if dt == F64 else np.float32)
+            buf = np.zeros(max(self.num_indices(), 1), _NP_VEC[dt])
         _check(lib().spmvx_l2gmap_update(self._h, dt, _dptr(vec), _np_ptr(buf), _stream(stream)))
         return None if buf is None else buf[: self.num_indices()]
 
     def reverse_update(self, vec, incoming=None, stream=None):
         """L2GMap::reverse_update: returns the ghost values sent to the owners
         and scatter-adds `incoming` (host, |indexbuf| values) into vec."""
-        import torch
-        dt = F64 if vec.dtype == torch.float64 else F32
+        dt = _vec_dtype_of(vec)
         if vec.numel() != self.local_size() + self.num_ghosts():
             raise SpmvError("vector length differs from local_size + num_ghosts")
-        npdt = np.float64 if dt == F64 else np.float32
+        npdt = _NP_VEC[dt]
         out = np.zeros(max(self.num_ghosts(), 1), npdt)
         inc = None if incoming is None else np.ascontiguousarray(incoming, npdt)
         if inc is not None and len(inc) != self.num_indices():
@@ -292,7 +302,10 @@ class Matrix:
         else:
             rowptr = np.ascontiguousarray(rowptr, np.int32)
             colidx = np.ascontiguousarray(colidx, np.int32)
-            values = np.ascontiguousarray(values, np.float64)
+            if _DTYPES[dtype] in (C64, C32):  # (re, im) pairs of doubles
+                values = np.ascontiguousarray(values, np.complex128).view(np.float64)
+            else:
+                values = np.ascontiguousarray(values, np.float64)
             nrows = len(rowptr) - 1
             if ncols is None:
                 ncols = col_map.local_size() + col_map.num_ghosts() if col_map else nrows

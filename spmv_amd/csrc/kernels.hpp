@@ -31,6 +31,43 @@ struct bf16_t
   uint16_t bits;
 };
 
+// Complex values (Matrix<std::complex<T>>, Matrix.cpp:441-444): the product
+// and sum std::complex evaluates without FMA, re = ar xr - ai xi,
+// im = ar xi + ai xr, every product and sum rounded separately.
+template <typename T>
+struct cplx
+{
+  T re, im;
+  cplx() = default;
+  __host__ __device__ constexpr cplx(T r, T i = T(0)) : re(r), im(i) {}
+};
+template <typename T>
+__host__ __device__ inline cplx<T> operator*(cplx<T> a, cplx<T> b)
+{
+  return cplx<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename T>
+__host__ __device__ inline cplx<T> operator+(cplx<T> a, cplx<T> b)
+{
+  return cplx<T>(a.re + b.re, a.im + b.im);
+}
+template <typename T>
+__host__ __device__ inline cplx<T> operator-(cplx<T> a, cplx<T> b)
+{
+  return cplx<T>(a.re - b.re, a.im - b.im);
+}
+template <typename T>
+__host__ __device__ inline cplx<T>& operator+=(cplx<T>& a, cplx<T> b)
+{
+  a.re += b.re;
+  a.im += b.im;
+  return a;
+}
+template <typename T>
+constexpr bool kIsComplex = false;
+template <typename T>
+constexpr bool kIsComplex<cplx<T>> = true;
+
 
 // ---------------------------------------------------------------- CG state
 // Scalars of one CG solve, resident on the device.  Every kernel of an
@@ -117,6 +154,11 @@ template <bool NT>
 __device__ __forceinline__ double ld_val(const double* v, int64_t k) { return ld_stream<NT>(v + k); }
 template <bool NT>
 __device__ __forceinline__ float ld_val(const float* v, int64_t k) { return ld_stream<NT>(v + k); }
+template <bool NT, typename T>
+__device__ __forceinline__ cplx<T> ld_val(const cplx<T>* v, int64_t k)
+{
+  return v[k];
+}
 template <bool NT>
 __device__ __forceinline__ float ld_val(const bf16_t* v, int64_t k)
 {
@@ -168,6 +210,12 @@ template <typename T>
 __device__ __forceinline__ void store_result(T* p, T v)
 {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void store_result(cplx<T>* p, cplx<T> v)
+{
+  store_result(&p->re, v.re);
+  store_result(&p->im, v.im);
 }
 
 template <typename V>
@@ -335,11 +383,12 @@ __global__ __launch_bounds__(BS) void csr_stream(
     }
   }
 
-  if (MODE == kPlain)
-    return;
-  part = block_sum<Acc, true>(part, red);
-  if (tid == 0)
-    partials[bi] = part;
+  if constexpr (MODE != kPlain)
+  {
+    part = block_sum<Acc, true>(part, red);
+    if (tid == 0)
+      partials[bi] = part;
+  }
 }
 
 // Software-pipelined variant.  A fixed grid of workgroups (about as many as
@@ -616,7 +665,9 @@ int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t
   int64_t grid = nblocks;
 #define SPMVX_CASE(i, K, C, VV, U, N, B)                                                     \
   case i:                                                                                    \
-    if constexpr (K == 0)                                                                    \
+    if constexpr (kIsComplex<V> && (K != 0 || C > 2048))                                     \
+      throw Error(SPMVX_ERR_ARG, "kernel variant not available for complex values");         \
+    else if constexpr (K == 0)                                                               \
       csr_stream<V, X, Acc, MODE, C, VV, U, N, B><<<dim3(static_cast<unsigned>(grid)),       \
                                                     dim3(B), 0, s>>>(                       \
           rowptr, colidx, val, x, y, blocks, nblocks, b, partials, st, it);                  \

@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <type_traits>
+
 #include "handles.hpp"
 
 namespace spmvx
@@ -62,7 +64,23 @@ size_t vec_elem(int32_t dtype)
 {
   if (dtype == SPMVX_F64) return 8;
   if (dtype == SPMVX_F32) return 4;
-  throw Error(SPMVX_ERR_ARG, "vectors must be F64 or F32");
+  if (dtype == SPMVX_C64) return 16;
+  if (dtype == SPMVX_C32) return 8;
+  throw Error(SPMVX_ERR_ARG, "vectors must be F64, F32, C64 or C32");
+}
+
+// Calls f with a null pointer of the element type of dtype.
+template <typename F>
+void with_vec_type(int32_t dtype, F&& f)
+{
+  switch (dtype)
+  {
+  case SPMVX_F64: f(static_cast<double*>(nullptr)); break;
+  case SPMVX_F32: f(static_cast<float*>(nullptr)); break;
+  case SPMVX_C64: f(static_cast<cplx<double>*>(nullptr)); break;
+  case SPMVX_C32: f(static_cast<cplx<float>*>(nullptr)); break;
+  default: throw Error(SPMVX_ERR_ARG, "vectors must be F64, F32, C64 or C32");
+  }
 }
 
 } // namespace
@@ -75,7 +93,7 @@ void l2gmap_prepare_device(spmvx_l2gmap_s* m)
   m->d_indexbuf.ensure(4 * std::max<size_t>(nidx, 1));
   if (nidx)
     SPMVX_HIP(hipMemcpy(m->d_indexbuf.ptr, m->indexbuf.data(), 4 * nidx, hipMemcpyHostToDevice));
-  m->d_send.ensure(8 * std::max<size_t>(nidx, 1));
+  m->d_send.ensure(16 * std::max<size_t>(nidx, 1)); // up to complex double
   const size_t nt = m->rev_target.size();
   m->d_rev_target.ensure(4 * std::max<size_t>(nt, 1));
   m->d_rev_ptr.ensure(4 * (nt + 1));
@@ -108,14 +126,12 @@ void l2gmap_update_async(spmvx_l2gmap_s* m, int32_t dtype, void* vec, hipStream_
     inbox = m->d_halo_view;
   }
   const int g = grid_for(std::max(nidx, ng));
-  if (dtype == SPMVX_F64)
-    halo_pack_unpack<double><<<g, 256, 0, s>>>(m->d_indexbuf.as<int32_t>(), nidx,
-                                                 static_cast<double*>(vec), m->d_send.as<double>(),
-                                                 static_cast<const double*>(inbox), m->local_size, ng);
-  else
-    halo_pack_unpack<float><<<g, 256, 0, s>>>(m->d_indexbuf.as<int32_t>(), nidx,
-                                                static_cast<float*>(vec), m->d_send.as<float>(),
-                                                static_cast<const float*>(inbox), m->local_size, ng);
+  with_vec_type(dtype, [&](auto* tag) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    halo_pack_unpack<T><<<g, 256, 0, s>>>(m->d_indexbuf.as<int32_t>(), nidx, static_cast<T*>(vec),
+                                          m->d_send.as<T>(), static_cast<const T*>(inbox),
+                                          m->local_size, ng);
+  });
   check_launch("halo_pack_unpack");
 }
 
@@ -381,14 +397,12 @@ int spmvx_l2gmap_reverse_update(spmvx_l2gmap m, int32_t dtype, void* vec,
       SPMVX_HIP(hipMemcpyAsync(m->d_send.ptr, m->h_in.ptr, es * nidx, hipMemcpyHostToDevice, s));
       const int64_t nt = static_cast<int64_t>(m->rev_target.size());
       const int g = grid_for(nt);
-      if (dtype == SPMVX_F64)
-        reverse_scatter_add<double><<<g, 256, 0, s>>>(
+      with_vec_type(dtype, [&](auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        reverse_scatter_add<T><<<g, 256, 0, s>>>(
             m->d_rev_target.as<int32_t>(), m->d_rev_ptr.as<int32_t>(), m->d_rev_pos.as<int32_t>(),
-            nt, m->d_send.as<double>(), static_cast<double*>(vec));
-      else
-        reverse_scatter_add<float><<<g, 256, 0, s>>>(
-            m->d_rev_target.as<int32_t>(), m->d_rev_ptr.as<int32_t>(), m->d_rev_pos.as<int32_t>(),
-            nt, m->d_send.as<float>(), static_cast<float*>(vec));
+            nt, m->d_send.as<T>(), static_cast<T*>(vec));
+      });
       check_launch("reverse_scatter_add");
     }
     SPMVX_HIP(hipStreamSynchronize(s));

@@ -204,8 +204,10 @@ int default_variant(int32_t dtype)
     const int x = std::atoi(e);
     return (x >= 0 && x < kNumVariants) ? x : -1;
   }();
-  if (forced >= 0)
+  if (forced >= 0 && dtype != SPMVX_C64 && dtype != SPMVX_C32)
     return forced;
+  if (dtype == SPMVX_C64 || dtype == SPMVX_C32) // variants 0-4 only (16 KB+ LDS products)
+    return forced >= 0 && forced <= 4 ? forced : 2;
   return dtype == SPMVX_F64 ? 11 : (dtype == SPMVX_F32 ? 3 : 4);
 }
 
@@ -246,6 +248,8 @@ __device__ __forceinline__ uint16_t dev_f64_to_bf16(double d)
   return static_cast<uint16_t>(u >> 16);
 }
 
+bool is_complex(int32_t dtype) { return dtype == SPMVX_C64 || dtype == SPMVX_C32; }
+
 size_t dtype_size(int32_t dtype)
 {
   switch (dtype)
@@ -253,6 +257,8 @@ size_t dtype_size(int32_t dtype)
   case SPMVX_F64: return 8;
   case SPMVX_F32: return 4;
   case SPMVX_BF16: return 2;
+  case SPMVX_C64: return 16;
+  case SPMVX_C32: return 8;
   default: throw Error(SPMVX_ERR_ARG, "unknown dtype");
   }
 }
@@ -260,13 +266,15 @@ size_t dtype_size(int32_t dtype)
 __global__ void convert_values(const double* __restrict__ src, void* __restrict__ dst,
                                int64_t n, int32_t dtype)
 {
+  // n values; complex dtypes read and write (re, im) pairs
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+  const int64_t m = (dtype == SPMVX_C64 || dtype == SPMVX_C32) ? 2 * n : n;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < m; i += stride)
   {
     const double d = src[i];
-    if (dtype == SPMVX_F64)
+    if (dtype == SPMVX_F64 || dtype == SPMVX_C64)
       static_cast<double*>(dst)[i] = d;
-    else if (dtype == SPMVX_F32)
+    else if (dtype == SPMVX_F32 || dtype == SPMVX_C32)
       static_cast<float*>(dst)[i] = static_cast<float>(d);
     else
       static_cast<bf16_t*>(dst)[i].bits = dev_f64_to_bf16(d);
@@ -359,8 +367,12 @@ void launch_spmv(int32_t dtype, int var, Pattern& P, const void* vals, const voi
     launch_plain<double, double, double>(var, P, vals, x, y, s);
   else if (dtype == SPMVX_F32)
     launch_plain<float, float, float>(var, P, vals, x, y, s);
-  else
+  else if (dtype == SPMVX_BF16)
     launch_plain<bf16_t, float, float>(var, P, vals, x, y, s);
+  else if (dtype == SPMVX_C64)
+    launch_plain<cplx<double>, cplx<double>, cplx<double>>(var, P, vals, x, y, s);
+  else
+    launch_plain<cplx<float>, cplx<float>, cplx<float>>(var, P, vals, x, y, s);
 }
 
 // Fills A->svalues from A->values on stream s (layout built beforehand).
@@ -457,6 +469,7 @@ void build_transpose(spmvx_matrix_s* A)
 
 SlicedView sliced_view(spmvx_matrix_s* A)
 {
+  SPMVX_REQUIRE(!is_complex(A->dtype), "the sliced layout is for real values");
   Pattern& P = *A->pat;
   const int vw = static_cast<int>(8 / dtype_size(A->dtype));
   const RowPatterns* R = P.row_patterns();
@@ -543,12 +556,12 @@ int spmvx_matrix_create(int32_t dtype, int64_t nrows, int64_t ncols, int64_t nnz
     A->values.ensure(es * static_cast<size_t>(nnz + 16));
     if (nnz > 0)
     {
-      if (dtype == SPMVX_F64)
-        SPMVX_HIP(hipMemcpy(A->values.ptr, values, 8 * nnz, hipMemcpyHostToDevice));
-      else if (dtype == SPMVX_F32)
+      if (dtype == SPMVX_F64 || dtype == SPMVX_C64)
+        SPMVX_HIP(hipMemcpy(A->values.ptr, values, es * nnz, hipMemcpyHostToDevice));
+      else if (dtype == SPMVX_F32 || dtype == SPMVX_C32)
       {
-        std::vector<float> f(values, values + nnz);
-        SPMVX_HIP(hipMemcpy(A->values.ptr, f.data(), 4 * nnz, hipMemcpyHostToDevice));
+        std::vector<float> f(values, values + (es / 4) * nnz);
+        SPMVX_HIP(hipMemcpy(A->values.ptr, f.data(), es * nnz, hipMemcpyHostToDevice));
       }
       else
       {
@@ -640,6 +653,8 @@ int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant)
   return guarded([&] {
     SPMVX_REQUIRE(A != nullptr, "NULL matrix");
     SPMVX_REQUIRE(variant >= -1 && variant <= kSlicedVariant, "unknown kernel variant");
+    SPMVX_REQUIRE(!is_complex(A->dtype) || variant <= 4,
+                  "kernel variant not available for complex values (use 0-4)");
     if (variant == kSlicedVariant)
       (void)sliced_view(A); // builds the layout or reports why it does not apply
     A->variant = variant < 0 ? default_variant(A->dtype) : variant;
@@ -678,6 +693,7 @@ int spmvx_matrix_sliced_info(spmvx_matrix A, int32_t* npatterns, int32_t* wmax, 
 {
   return guarded([&] {
     SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    SPMVX_REQUIRE(!is_complex(A->dtype), "the sliced layout is for real values");
     Pattern& P = *A->pat;
     const RowPatterns* R = P.row_patterns();
     SPMVX_REQUIRE(R != nullptr, P.rpat_error);

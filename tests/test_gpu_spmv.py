@@ -270,3 +270,85 @@ def test_sliced_layout_set_values():
     Mc = S.Matrix(A.rowptr, A.colidx, v2, ncols=A.ncols, dtype="bf16")
     xf = _dev(x, torch.float32)
     assert np.array_equal(Mb.mult(xf).cpu().numpy(), Mc.mult(xf).cpu().numpy())
+
+
+# ---------------------------------------------------------------- complex values
+def _cdev(a, dtype=torch.complex128):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=DEV, dtype=dtype)
+
+
+@pytest.mark.parametrize("name,A", [m for m in MATS if m[0] in
+                                    ("poisson7", "stencil27_ghost", "random_spd", "long_rows", "one_row")],
+                         ids=["poisson7", "stencil27_ghost", "random_spd", "long_rows", "one_row"])
+def test_complex_spmv_bitexact(name, A):
+    """Matrix<std::complex<double>> / <float> products: bit-identical to the
+    oracle's sequential complex row sums (which equal scipy's), every kernel
+    variant that takes complex values, and the transposed product."""
+    v = A.val + 1j * P.uniform(13, A.nnz, -1, 1)
+    x = P.uniform(14, A.ncols, -1, 1) + 1j * P.uniform(15, A.ncols, -1, 1)
+    oc = _ocsr(A)
+    want = O.spmv_complex(oc, v, x)
+    M = S.Matrix(A.rowptr, A.colidx, v, ncols=A.ncols, dtype="c64")
+    assert M.vector_dtype() == torch.complex128
+    xd = _cdev(x)
+    for variant in range(5):
+        M.set_kernel(variant)
+        assert np.array_equal(M.mult(xd).cpu().numpy(), want), variant
+    xt = P.uniform(16, A.nrows, -1, 1) + 1j * P.uniform(17, A.nrows, -1, 1)
+    assert np.array_equal(M.transpmult(_cdev(xt)).cpu().numpy(), O.spmv_t_complex(oc, v, xt))
+    M32 = S.Matrix(A.rowptr, A.colidx, v, ncols=A.ncols, dtype="c32")
+    x32 = x.astype(np.complex64)
+    want32 = O.spmv_complex(oc, v.astype(np.complex64), x32)
+    assert np.array_equal(M32.mult(_cdev(x32, torch.complex64)).cpu().numpy(), want32)
+
+
+def test_complex_restrictions_and_values():
+    A = MATS[2][1]  # poisson7
+    v = A.val + 0.5j
+    M = S.Matrix(A.rowptr, A.colidx, v, ncols=A.ncols, dtype="c64")
+    for bad in (5, 11, S.SLICED_VARIANT):
+        with pytest.raises(S.SpmvError, match="complex|real values"):
+            M.set_kernel(bad)
+    with pytest.raises(S.SpmvError, match="real values"):
+        M.sliced_info()
+    with pytest.raises(S.SpmvError):
+        S.cg(M, _cdev(np.ones(A.nrows)), 10, 1e-8)
+    # new values from the device ((re, im) pairs)
+    v2 = P.uniform(18, A.nnz, -1, 1) - 2j * P.uniform(19, A.nnz, -1, 1)
+    M.set_values(_cdev(v2))
+    x = P.uniform(20, A.ncols, -1, 1) + 1j * P.uniform(21, A.ncols, -1, 1)
+    assert np.array_equal(M.mult(_cdev(x)).cpu().numpy(), O.spmv_complex(_ocsr(A), v2, x))
+
+
+@pytest.mark.parametrize("tdt", [torch.complex128, torch.complex64])
+def test_complex_update_and_reverse(tdt):
+    """L2GMap<std::complex<T>>::update / reverse_update (L2GMap.cpp:202-213):
+    pure data movement and ordered scatter-add on complex entries."""
+    import json, os
+    gold = [json.load(open(os.path.join(os.path.dirname(__file__), "golden", "l2gmap",
+                                        f"scatter_3_{r}.json"))) for r in range(3)]
+    rank = 1
+    g = gold[rank]
+    lo, hi = g["ranges"][rank], g["ranges"][rank + 1]
+    remote = {q: [v for v in gold[q]["ghosts"] if lo <= v < hi] for q in range(3) if q != rank}
+    m = S.L2GMap(g["local_size"], g["ghosts"], local_sizes=[x["local_size"] for x in gold],
+                 rank=rank, remote_ghosts=remote)
+    n, ng = m.local_size(), m.num_ghosts()
+    npdt = np.complex128 if tdt == torch.complex128 else np.complex64
+    v = (P.uniform(22, n + ng, -1, 1) + 1j * P.uniform(23, n + ng, -1, 1)).astype(npdt)
+    halo = (P.uniform(24, ng, -1, 1) - 1j * P.uniform(25, ng, -1, 1)).astype(npdt)
+    m.set_halo(halo, dtype=S.C64 if tdt == torch.complex128 else S.C32)
+    vd = _cdev(v, tdt)
+    send = m.update(vd, send_out=True)
+    idx = m.plan()["indexbuf"]
+    want = v.copy()
+    want[n:] = halo
+    assert np.array_equal(vd.cpu().numpy(), want)
+    assert np.array_equal(send, v[np.asarray(idx, np.int64)])
+    inc = (P.uniform(26, len(idx), -1, 1) + 1j * P.uniform(27, len(idx), -1, 1)).astype(npdt)
+    out = m.reverse_update(vd, inc)
+    assert np.array_equal(out, halo)
+    acc = want.copy()
+    for j, t in enumerate(idx):  # sequential order of L2GMap.cpp:168-169
+        acc[t] = acc[t] + inc[j]
+    assert np.array_equal(vd.cpu().numpy()[:n], acc[:n])

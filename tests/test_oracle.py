@@ -345,3 +345,27 @@ def diffusion_edge_nodes(n, e):
         return i + n * j, i + n * (j + 1)
     side, t = divmod(e - 2 * nh, n)
     return [t, t + n * (n - 1), n * t, n - 1 + n * t][side], None
+
+
+# ---------------------------------------------------------------- complex values
+@pytest.mark.parametrize("A", [O.gen_stencil(7, 9, 8, 7), O.gen_stencil(27, 5, 5, 5, ghost_planes=1),
+                               O.gen_demo1d(50)], ids=["poisson7", "stencil27_ghost", "demo1d"])
+def test_complex_spmv_bitexact_vs_scipy(A):
+    """Matrix<std::complex<T>> (Matrix.cpp:441-444): the row order of the
+    real product with std::complex's product; scipy's csr_matvec evaluates
+    the same formula in the same order, so the results are identical."""
+    import scipy.sparse as sp
+    v = A.val + 1j * O.uniform(3, A.nnz, -1, 1)
+    x = O.uniform(4, A.ncols, -1, 1) + 1j * O.uniform(5, A.ncols, -1, 1)
+    S = sp.csr_matrix((v, A.colidx, A.rowptr), shape=(A.nrows, A.ncols))
+    y = O.spmv_complex(A, v, x)
+    assert np.array_equal(y, S @ x)
+    assert np.allclose(y, S.toarray() @ x, rtol=1e-13, atol=1e-13)
+    v32, x32 = v.astype(np.complex64), x.astype(np.complex64)
+    S32 = sp.csr_matrix((v32, A.colidx, A.rowptr), shape=(A.nrows, A.ncols))
+    assert np.array_equal(O.spmv_complex(A, v32, x32), S32 @ x32)
+    xt = x[:A.nrows]
+    assert np.array_equal(O.spmv_t_complex(A, v, xt), S.T @ xt)
+    # a real matrix as complex: the real part is the real product
+    assert np.array_equal(O.spmv_complex(A, A.val.astype(np.complex128), x.real + 0j).real,
+                          O.spmv(A, x.real))
