@@ -1,6 +1,7 @@
 // spmv::Matrix<T> -- distributed CSR matrix block (the interface of
 // spmv/Matrix.h:23-81) stored and multiplied on the device, backed by the
-// spmvx C ABI.  T = double (FP64) or float (FP32).
+// spmvx C ABI.  T = double, float, std::complex<double> or
+// std::complex<float> (the reference's instantiations, Matrix.cpp:441-444).
 #pragma once
 
 #include <memory>

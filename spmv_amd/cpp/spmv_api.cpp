@@ -3,6 +3,7 @@
 // message, which for the reference's checks is the reference's text.
 #include <hip/hip_runtime.h>
 
+#include <complex>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -30,7 +31,32 @@ void hip_check(hipError_t e, const char* what)
 template <typename T>
 constexpr int32_t dtype_of()
 {
-  return std::is_same<T, double>::value ? SPMVX_F64 : SPMVX_F32;
+  if (std::is_same<T, double>::value)
+    return SPMVX_F64;
+  if (std::is_same<T, std::complex<double>>::value)
+    return SPMVX_C64;
+  if (std::is_same<T, std::complex<float>>::value)
+    return SPMVX_C32;
+  return SPMVX_F32;
+}
+
+// Host values of a CSR as the doubles the C ABI takes ((re, im) pairs for
+// complex values).
+template <typename T>
+std::vector<double> as_doubles(const std::vector<T>& v)
+{
+  return std::vector<double>(v.begin(), v.end());
+}
+template <typename R>
+std::vector<double> as_doubles(const std::vector<std::complex<R>>& v)
+{
+  std::vector<double> d(2 * v.size());
+  for (std::size_t i = 0; i < v.size(); ++i)
+  {
+    d[2 * i] = static_cast<double>(v[i].real());
+    d[2 * i + 1] = static_cast<double>(v[i].imag());
+  }
+  return d;
 }
 
 int comm_size(MPI_Comm comm)
@@ -131,6 +157,8 @@ void Vector<T>::from_host(const std::vector<T>& host)
 
 template class Vector<double>;
 template class Vector<float>;
+template class Vector<std::complex<double>>;
+template class Vector<std::complex<float>>;
 
 // ---------------------------------------------------------------- L2GMap
 L2GMap::L2GMap(MPI_Comm comm, std::int64_t local_size, const std::vector<std::int64_t>& ghosts)
@@ -239,6 +267,15 @@ template void L2GMap::reverse_update<double>(double*, const std::vector<double>&
 template void L2GMap::reverse_update<float>(float*, const std::vector<float>&) const;
 template void L2GMap::set_halo<double>(const std::vector<double>&);
 template void L2GMap::set_halo<float>(const std::vector<float>&);
+// complex instantiations (L2GMap.cpp:202-213)
+template void L2GMap::update<std::complex<double>>(std::complex<double>*) const;
+template void L2GMap::update<std::complex<float>>(std::complex<float>*) const;
+template void L2GMap::reverse_update<std::complex<double>>(
+    std::complex<double>*, const std::vector<std::complex<double>>&) const;
+template void L2GMap::reverse_update<std::complex<float>>(
+    std::complex<float>*, const std::vector<std::complex<float>>&) const;
+template void L2GMap::set_halo<std::complex<double>>(const std::vector<std::complex<double>>&);
+template void L2GMap::set_halo<std::complex<float>>(const std::vector<std::complex<float>>&);
 
 // ---------------------------------------------------------------- Matrix
 template <typename T>
@@ -253,7 +290,7 @@ Matrix<T>::Matrix(const CSR<T>& A, std::shared_ptr<L2GMap> col_map,
     throw std::runtime_error("spmv::Matrix: inconsistent CSR arrays");
   if (_col_map && _col_map->local_size() + _col_map->num_ghosts() != A.cols)
     throw std::runtime_error("spmv::Matrix: column map size differs from the column count");
-  std::vector<double> v(A.values.begin(), A.values.end());
+  const std::vector<double> v = as_doubles(A.values);
   check(spmvx_matrix_create(dtype_of<T>(), A.rows, A.cols, _nnz, A.outer.data(), A.inner.data(),
                             v.data(), &_h));
 }
@@ -295,6 +332,8 @@ Vector<T> Matrix<T>::transpmult(const Vector<T>& b) const
 
 template class Matrix<double>;
 template class Matrix<float>;
+template class Matrix<std::complex<double>>; // Matrix.cpp:441-444
+template class Matrix<std::complex<float>>;
 
 // ---------------------------------------------------------------- cg
 std::tuple<Vector<double>, int> cg(MPI_Comm comm, const Matrix<double>& A,

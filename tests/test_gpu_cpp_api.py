@@ -50,3 +50,35 @@ def test_cgmain_matches_oracle():
     assert abs(field(out, "iterations") - k) <= 1
     assert field(out, "rel.residual") <= 1e-7
     assert abs(field(out, "x.norm") - np.linalg.norm(x)) <= 1e-8 * np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("mode", ["real", "complex"])
+def test_restrictmain_petsc(tmp_path, mode):
+    """demos/restrictmain.cpp's loop (transpmult, reverse_update, update,
+    product) through the C++ API on a PETSc file, real and
+    Matrix<std::complex<double>>, against dense numpy."""
+    import oracle.petsc as OP
+    rng = np.random.default_rng(9)
+    nr, nc = 40, 90
+    rows = [np.sort(rng.choice(nc, 6, replace=False)) for _ in range(nr)]
+    rowptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])])
+    colidx = np.concatenate(rows)
+    val = P.uniform(31, len(colidx), -0.3, 0.3)
+    OP.write_matrix(str(tmp_path / "R.dat"), nr, nc, rowptr, colidx, val)
+    b = P.uniform(32, nr, -1, 1)
+    OP.write_vector(str(tmp_path / "b.dat"), b)
+    args = [str(tmp_path / "R.dat"), str(tmp_path / "b.dat")] + (["complex"] if mode == "complex" else [])
+    out = run("restrictmain", *args)
+    R = np.zeros((nr, nc), complex)
+    for i in range(nr):
+        R[i, rows[i]] = val[rowptr[i]:rowptr[i + 1]]
+    q = b.astype(complex)
+    if mode == "complex":
+        R, q = R * (1 + 0.5j), q * (1 - 0.25j)
+    for _ in range(10):
+        p = R.T @ q
+        pn = float(np.sum(np.abs(p) ** 2))
+        q = R @ p
+        qn = float(np.sum(np.abs(q) ** 2))
+    assert abs(field(out, "pnorm") - pn) <= 1e-12 * pn
+    assert abs(field(out, "qnorm") - qn) <= 1e-12 * qn
