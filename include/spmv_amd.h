@@ -179,7 +179,7 @@ int spmvx_matrix_arrays(spmvx_matrix A, const int32_t** rowptr_dev,
  * 65535 patterns, rows over 256 entries, a pattern table over 8192 entries)
  * returns SPMVX_ERR_ARG with the reason and keeps its variant.  Transposed
  * products use the default CSR variant. */
-#define SPMVX_SLICED_VARIANT 14
+#define SPMVX_SLICED_VARIANT 15
 int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant);
 int spmvx_num_kernels(void);
 /* The variant A uses and a short description of its kernel (written into

@@ -29,7 +29,7 @@ __all__ = ["L2GMap", "Matrix", "Batch", "cg", "cg_f32", "ir_cg", "sddmm", "copy_
 
 F64, F32, BF16, C64, C32 = 0, 1, 2, 3, 4
 # kernel variant of the sliced row-pattern layout (SPMVX_SLICED_VARIANT)
-SLICED_VARIANT = 14
+SLICED_VARIANT = 15
 _DTYPES = {"f64": F64, "float64": F64, "double": F64, F64: F64,
            "f32": F32, "float32": F32, "float": F32, F32: F32,
            "bf16": BF16, "bfloat16": BF16, BF16: BF16,

@@ -407,27 +407,28 @@ struct PipeStage
   int32_t ra, re;
 };
 
-template <bool NT, typename V, int VEC, int PER>
+template <bool NT, typename V, int VEC, int PER, int BS = kBS>
 __device__ __forceinline__ void pipe_issue(PipeStage<V, VEC, PER>& S,
                                            const int32_t* __restrict__ rowptr,
                                            const int32_t* __restrict__ colidx,
                                            const V* __restrict__ val, int2 d0, int2 d1,
                                            int long_limit)
 {
+  // Every lane issues the same loads whatever the block (rows past the block
+  // read the first row's extents, a long row's leading entries are loaded
+  // and not used): with no branch around them, the compiler's wait for the
+  // current block's gathers (vmcnt counts loads in issue order) leaves these
+  // loads in flight.  d0.x < d1.x: every block holds a row.
+  (void)long_limit;
   const int tid = threadIdx.x;
-  const int32_t row = d0.x + tid;
-  if (row < d1.x)
-  {
-    S.ra = rowptr[row];
-    S.re = rowptr[row + 1];
-  }
-  if (d1.y - d0.y > long_limit)
-    return; // long row: loaded directly when processed
+  const int32_t row = d0.x + tid < d1.x ? d0.x + tid : d0.x;
+  S.ra = rowptr[row];
+  S.re = rowptr[row + 1];
   const int32_t kb = d0.y & ~(VEC - 1);
 #pragma unroll
   for (int u = 0; u < PER; ++u)
   {
-    const int32_t g = kb + (u * kBS + tid) * VEC;
+    const int32_t g = kb + (u * BS + tid) * VEC;
     const int32_t gs = g < d1.y ? g : kb; // stay inside the arrays, no branch
     load_vec<NT>(colidx + gs, S.c[u]);
     load_vec<NT>(val + gs, S.v[u]);
@@ -442,18 +443,14 @@ __device__ __forceinline__ void pipe_gather(const PipeStage<V, VEC, PER>& S,
                                             const X* __restrict__ x, int2 d0, int2 d1,
                                             X (&xg)[PER][VEC], int long_limit)
 {
-  const int32_t k0 = d0.y, k1 = d1.y;
-  if (k1 - k0 > long_limit)
-    return;
-  const int32_t kb = k0 & ~(VEC - 1);
+  (void)d0;
+  (void)d1;
+  (void)long_limit;
 #pragma unroll
   for (int u = 0; u < PER; ++u)
-  {
-    const int32_t g = kb + (u * kBS + static_cast<int32_t>(threadIdx.x)) * VEC;
 #pragma unroll
     for (int j = 0; j < VEC; ++j)
-      xg[u][j] = x[S.c[u][j]]; // entries outside the block hold valid columns
-  }
+      xg[u][j] = x[S.c[u][j]]; // every loaded column is valid (see pipe_issue)
 }
 
 template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, int PER, bool NT>
@@ -562,37 +559,27 @@ __global__ __launch_bounds__(kBS) void csr_pipe(
   X xg[PER][VEC];
   if (bb < be)
   {
+    // block bi is [a0, a1); a2 ends block bi+1 (loaded one iteration ahead);
+    // the next block's loads are unconditional (the current block again
+    // after the last one), see pipe_issue
     int2 a0 = blocks[bb], a1 = blocks[bb + 1];
+    int2 a2 = blocks[bb + 2 < be ? bb + 2 : be];
     pipe_issue<NT>(A, rowptr, colidx, val, a0, a1, kLong);
     int64_t bi = bb;
-    for (;;)
+    auto step = [&](PipeStage<V, VEC, PER>& cur, PipeStage<V, VEC, PER>& nxt) {
+      pipe_gather(cur, x, a0, a1, xg, kLong);
+      const bool more = bi + 1 < be;
+      pipe_issue<NT>(nxt, rowptr, colidx, val, more ? a1 : a0, more ? a2 : a1, kLong);
+      const int2 a3 = blocks[bi + 3 < be ? bi + 3 : be];
+      pipe_compute<V, X, Acc, MODE, CHUNK, VEC, PER, NT>(cur, xg, colidx, val, x, y, b, a0, a1,
+                                                          prod, part, kLong);
+      a0 = a1;
+      a1 = a2;
+      a2 = a3;
+      return ++bi < be;
+    };
+    while (step(A, B) && step(B, A))
     {
-      int2 n0 = a1, n1 = a1;
-      pipe_gather(A, x, a0, a1, xg, kLong);
-      if (bi + 1 < be)
-      {
-        n1 = blocks[bi + 2];
-        pipe_issue<NT>(B, rowptr, colidx, val, n0, n1, kLong);
-      }
-      pipe_compute<V, X, Acc, MODE, CHUNK, VEC, PER, NT>(A, xg, colidx, val, x, y, b, a0, a1,
-                                                          prod, part, kLong);
-      if (++bi >= be)
-        break;
-      a0 = n0;
-      a1 = n1;
-      pipe_gather(B, x, a0, a1, xg, kLong);
-      if (bi + 1 < be)
-      {
-        n0 = a1;
-        n1 = blocks[bi + 2];
-        pipe_issue<NT>(A, rowptr, colidx, val, n0, n1, kLong);
-      }
-      pipe_compute<V, X, Acc, MODE, CHUNK, VEC, PER, NT>(B, xg, colidx, val, x, y, b, a0, a1,
-                                                          prod, part, kLong);
-      if (++bi >= be)
-        break;
-      a0 = n0;
-      a1 = n1;
     }
   }
   if (MODE == kPlain)
@@ -602,34 +589,279 @@ __global__ __launch_bounds__(kBS) void csr_pipe(
     partials[w] = part;
 }
 
+// Persistent variant that holds its results in LDS.  OCC workgroups per CU
+// (sharing its LDS) each walk a sequence of row blocks with the next block's
+// column indices, values and row extents in flight in registers, as
+// csr_pipe does; the row sums go to an LDS buffer that is written to y only
+// when it is full, and at the end.  On a 27-point share (8,000 rows per
+// workgroup, 7,872 held) the results are written in two bursts near the
+// end: a stream of
+// reads interleaved with a 2.4% trickle of writes ran ~20% slower than the
+// same reads with the writes held back and written last (probe kernels on
+// MI355X).  The workgroups of one XCD take that XCD's contiguous range of
+// blocks in turn (slot j: blocks j, j + slots, ...), so they move through
+// memory together (2.4% faster than a contiguous range per workgroup).
+// Per-row arithmetic is csr_stream's (bit-identical).
+// Measured on the 27-point share: 0.246 ms alone (csr_stream 0.255 ms), but
+// no faster inside update + product and 10-20% slower in CG, where the
+// static split cannot absorb the skew that the dynamic dispatch of
+// csr_stream does; it is a tuning option, not a default.
+constexpr int kLdsBytes = 160 * 1024; // LDS per CU on gfx950
+constexpr int kDeferRuns = 256;        // row runs the result buffer tracks
+
+// Rows the result buffer of csr_defer holds (a multiple of 64) when OCC
+// workgroups share a CU's LDS.
+template <typename X, typename Acc, int CHUNK, int OCC>
+constexpr int defer_ycap()
+{
+  return ((kLdsBytes / OCC - (CHUNK + 1 + 16) * static_cast<int>(sizeof(Acc))
+           - (kDeferRuns + 1) * 8 - 256)
+          / static_cast<int>(sizeof(X)))
+         & ~63;
+}
+
+// Result buffer of csr_defer: `fill` slots of ybuf hold runs of consecutive
+// rows; run k starts at row run[k].x and slot run[k].y.  Every thread keeps
+// the same copy of the counters.
+template <typename X>
+struct DeferBuf
+{
+  X* ybuf;
+  int2* run;
+  int32_t nrun, fill, last; // last = end row of the last run
+};
+
+// Writes the buffered rows to y and empties the buffer.  Called by all
+// threads after a barrier; ends with one (the buffer is reused next).
+template <typename X, int BS>
+__device__ __forceinline__ void defer_flush(DeferBuf<X>& D, X* __restrict__ y)
+{
+  const int tid = threadIdx.x;
+  constexpr int NW = BS / 64;
+  if (D.nrun >= NW)
+  {
+    // many short runs: one wavefront per run
+    for (int k = tid / 64; k < D.nrun; k += NW)
+    {
+      const int2 r = D.run[k];
+      const int32_t e = k + 1 < D.nrun ? D.run[k + 1].y : D.fill;
+      for (int32_t i = r.y + (tid & 63); i < e; i += 64)
+        y[r.x + (i - r.y)] = D.ybuf[i];
+    }
+  }
+  else
+  {
+    for (int k = 0; k < D.nrun; ++k)
+    {
+      const int2 r = D.run[k];
+      const int32_t e = k + 1 < D.nrun ? D.run[k + 1].y : D.fill;
+      for (int32_t i = r.y + tid; i < e; i += BS)
+        y[r.x + (i - r.y)] = D.ybuf[i];
+    }
+  }
+  D.nrun = 0;
+  D.fill = 0;
+  D.last = -1;
+  lds_barrier();
+}
+
+template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, int PER, int OCC,
+          int BS, bool NT>
+__device__ __forceinline__ void defer_compute(const PipeStage<V, VEC, PER>& S,
+                                              const X (&xg)[PER][VEC],
+                                              const int32_t* __restrict__ colidx,
+                                              const V* __restrict__ val,
+                                              const X* __restrict__ x, X* __restrict__ y,
+                                              X aux, int2 d0, int2 d1, Acc* prod,
+                                              DeferBuf<X>& D, Acc& part)
+{
+  // aux: x[row] (kDot) or b[row] (kResid) of this thread's row, loaded by the
+  // caller before the gathers (a load issued here would wait for the next
+  // block's loads, which retire first)
+  constexpr int YCAP = defer_ycap<X, Acc, CHUNK, OCC>();
+  constexpr int kLong = CHUNK - 8;
+  const int tid = threadIdx.x;
+  const int32_t r0 = d0.x, r1 = d1.x, k0 = d0.y, k1 = d1.y;
+  // rows held so far are complete (the previous block ended with a barrier)
+  if (D.fill + (r1 - r0) > YCAP || (D.nrun == kDeferRuns && r0 != D.last))
+    defer_flush<X, BS>(D, y);
+  if (r0 != D.last)
+  {
+    if (tid == 0)
+      D.run[D.nrun] = make_int2(r0, D.fill);
+    ++D.nrun;
+  }
+  X* yb = D.ybuf + D.fill; // slot of row r: yb[r - r0]
+  D.fill += r1 - r0;
+  D.last = r1;
+  const int32_t row = r0 + tid;
+  if (k1 - k0 <= kLong)
+  {
+    const int32_t kb = k0 & ~(VEC - 1);
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+    {
+      const int32_t g = kb + (u * BS + tid) * VEC;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j)
+      {
+        const int32_t k = g + j; // outside the block: spare slot prod[CHUNK]
+        const bool in = g < k1 && k >= k0 && k < k1;
+        prod[in ? k - k0 : CHUNK] =
+            static_cast<Acc>(widen(S.v[u][j])) * static_cast<Acc>(xg[u][j]);
+      }
+    }
+    lds_barrier();
+    if (row < r1)
+    {
+      const Acc s = row_sum(prod, S.ra - k0, S.re - k0);
+      const X yi = MODE == kResid ? aux - static_cast<X>(s) : static_cast<X>(s);
+      yb[row - r0] = yi;
+      if (MODE == kResid)
+        part += static_cast<Acc>(yi) * static_cast<Acc>(yi);
+      if (MODE == kDot)
+        part += static_cast<Acc>(aux) * static_cast<Acc>(yi);
+    }
+    lds_barrier();
+  }
+  else
+  {
+    // one long row, staged chunk by chunk and summed in order by thread 0
+    Acc s = Acc(0);
+    for (int64_t c0 = k0; c0 < k1; c0 += CHUNK)
+    {
+      const int n = static_cast<int>((k1 - c0) < CHUNK ? (k1 - c0) : CHUNK);
+      for (int t = tid; t < n; t += BS)
+      {
+        const int64_t k = c0 + t;
+        prod[t] = static_cast<Acc>(ld_val<NT>(val, k)) * static_cast<Acc>(x[colidx[k]]);
+      }
+      lds_barrier();
+      if (tid == 0)
+        for (int q = 0; q < n; ++q)
+          s += prod[q];
+      lds_barrier();
+    }
+    if (tid == 0) // row r0 is thread 0's row
+    {
+      const X yi = MODE == kResid ? aux - static_cast<X>(s) : static_cast<X>(s);
+      yb[0] = yi;
+      if (MODE == kResid)
+        part += static_cast<Acc>(yi) * static_cast<Acc>(yi);
+      if (MODE == kDot)
+        part += static_cast<Acc>(aux) * static_cast<Acc>(yi);
+    }
+    lds_barrier(); // the result is in the buffer before any thread flushes it
+  }
+}
+
+template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, bool NT, int BS,
+          int OCC>
+__global__ __launch_bounds__(BS, OCC* BS / 256) void csr_defer( // OCC workgroups' waves per SIMD
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+    const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
+    const int2* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
+    Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it)
+{
+  if (MODE != kPlain && !cg_live(st, it))
+    return;
+  constexpr int PER = CHUNK / (BS * VEC);
+  static_assert(PER * BS * VEC == CHUNK, "CHUNK must be a multiple of BS*VEC");
+  constexpr int kLong = CHUNK - 8;
+  __shared__ Acc prod[CHUNK + 1];
+  __shared__ X ybuf[defer_ycap<X, Acc, CHUNK, OCC>()];
+  __shared__ int2 run[kDeferRuns];
+  __shared__ Acc red[16];
+  const int tid = threadIdx.x;
+  // block sequence first, first + stride, ... < end: this XCD's share of the
+  // blocks (in proportion to its workgroups), taken in turn by its workgroups
+  const int64_t w = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t xcd = blockIdx.x % kNumXcd, slot = blockIdx.x / kNumXcd;
+  const int64_t q = gridDim.x / kNumXcd, rem = gridDim.x % kNumXcd;
+  const int64_t g0 = xcd * q + (xcd < rem ? xcd : rem); // first workgroup of this XCD
+  const int64_t stride = q + (xcd < rem ? 1 : 0);
+  const int64_t first = nblocks * g0 / gridDim.x + slot;
+  const int64_t end = nblocks * (g0 + stride) / gridDim.x;
+  Acc part = Acc(0);
+  DeferBuf<X> D{ybuf, run, 0, 0, -1};
+  if (first < end)
+  {
+    // Two register stages: the next block's loads are issued after the
+    // current block's gathers and stay in flight while the gathers return
+    // and the block is summed.  (More stages would not help: loads retire in
+    // issue order, so waiting for the gathers waits for every load issued
+    // before them.)  A step past the sequence reloads the current block, so
+    // every step issues the same loads.  Descriptors: the current block
+    // [c0, c1) and the next [n0, n1), loaded one step ahead.
+    PipeStage<V, VEC, PER> A, B;
+    X xg[PER][VEC];
+    int64_t bi = first;
+    int2 c0 = blocks[bi], c1 = blocks[bi + 1];
+    const int64_t nb = bi + stride < end ? bi + stride : bi;
+    int2 n0 = blocks[nb], n1 = blocks[nb + 1];
+    pipe_issue<NT, V, VEC, PER, BS>(A, rowptr, colidx, val, c0, c1, kLong);
+    auto step = [&](PipeStage<V, VEC, PER>& cur, PipeStage<V, VEC, PER>& nxt) {
+      X aux = X(0);
+      if (MODE != kPlain && c0.x + tid < c1.x)
+        aux = MODE == kDot ? x[c0.x + tid] : b[c0.x + tid];
+      pipe_gather(cur, x, c0, c1, xg, kLong);
+      const bool more = bi + stride < end;
+      pipe_issue<NT, V, VEC, PER, BS>(nxt, rowptr, colidx, val, more ? n0 : c0,
+                                      more ? n1 : c1, kLong);
+      const int64_t ab = bi + 2 * stride < end ? bi + 2 * stride : bi;
+      const int2 a0 = blocks[ab], a1 = blocks[ab + 1];
+      defer_compute<V, X, Acc, MODE, CHUNK, VEC, PER, OCC, BS, NT>(cur, xg, colidx, val, x, y,
+                                                                   aux, c0, c1, prod, D, part);
+      c0 = n0;
+      c1 = n1;
+      n0 = a0;
+      n1 = a1;
+      bi += stride;
+      return bi < end;
+    };
+    while (step(A, B) && step(B, A))
+    {
+    }
+    defer_flush<X, BS>(D, y);
+  }
+  if (MODE == kPlain)
+    return;
+  part = block_sum<Acc, true>(part, red);
+  if (threadIdx.x == 0)
+    partials[w] = part;
+}
+
 // Kernel variants selectable per matrix: {kind (0 one block per workgroup,
-// 1 pipelined), chunk, vec, unroll, non-temporal}.
+// 1 pipelined, 2 pipelined with results held in LDS), chunk, vec, unroll,
+// non-temporal, threads per workgroup}.
 struct SpmvVariant
 {
   int kind;
   int chunk;
   int vec;
-  int unroll;
+  int unroll; // kind 2: unused (1)
   bool nt;
-  int bs; // threads per workgroup = rows per block at most
+  int bs;  // threads per workgroup = rows per block at most
+  int occ; // kind 2: workgroups per CU (LDS shared between them)
 };
 #define SPMVX_VARIANTS(X)                                                                    \
-  X(0, 0, 2048, 1, 1, false, 256)                                                            \
-  X(1, 0, 2048, 1, 4, true, 256)                                                             \
-  X(2, 0, 2048, 2, 2, true, 256)                                                             \
-  X(3, 0, 2048, 4, 1, true, 256)                                                             \
-  X(4, 0, 2048, 4, 2, true, 256)                                                             \
-  X(5, 1, 2048, 2, 1, true, 256)                                                             \
-  X(6, 1, 2048, 4, 1, true, 256)                                                             \
-  X(7, 1, 2048, 8, 1, true, 256)                                                             \
-  X(8, 1, 4096, 4, 1, true, 256)                                                             \
-  X(9, 1, 1024, 4, 1, true, 256)                                                             \
-  X(10, 1, 2048, 4, 1, false, 256)                                                           \
-  X(11, 0, 8192, 2, 4, true, 1024)                                                           \
-  X(12, 0, 8192, 2, 2, true, 1024)                                                           \
-  X(13, 0, 8192, 4, 2, true, 1024)
-constexpr int kNumVariants = 14;
-#define SPMVX_VARIANT_ENTRY(i, K, C, V, U, N, B) {K, C, V, U, N, B},
+  X(0, 0, 2048, 1, 1, false, 256, 0)                                                         \
+  X(1, 0, 2048, 1, 4, true, 256, 0)                                                          \
+  X(2, 0, 2048, 2, 2, true, 256, 0)                                                          \
+  X(3, 0, 2048, 4, 1, true, 256, 0)                                                          \
+  X(4, 0, 2048, 4, 2, true, 256, 0)                                                          \
+  X(5, 1, 2048, 2, 1, true, 256, 0)                                                          \
+  X(6, 1, 2048, 4, 1, true, 256, 0)                                                          \
+  X(7, 1, 2048, 8, 1, true, 256, 0)                                                          \
+  X(8, 1, 4096, 4, 1, true, 256, 0)                                                          \
+  X(9, 1, 1024, 4, 1, true, 256, 0)                                                          \
+  X(10, 1, 2048, 4, 1, false, 256, 0)                                                        \
+  X(11, 0, 8192, 2, 4, true, 1024, 0)                                                        \
+  X(12, 0, 8192, 2, 2, true, 1024, 0)                                                        \
+  X(13, 0, 8192, 4, 2, true, 1024, 0)                                                        \
+  X(14, 2, 2048, 2, 1, true, 1024, 2)
+constexpr int kNumVariants = 15;
+#define SPMVX_VARIANT_ENTRY(i, K, C, V, U, N, B, O) {K, C, V, U, N, B, O},
 constexpr SpmvVariant kVariants[kNumVariants] = {SPMVX_VARIANTS(SPMVX_VARIANT_ENTRY)};
 #undef SPMVX_VARIANT_ENTRY
 
@@ -640,7 +872,7 @@ inline int variant_block_nnz(int var)
 {
   if (var < 0 || var >= kNumVariants)
     var = 0;
-  return kVariants[var].kind == 1 ? kVariants[var].chunk - 8 : kVariants[var].chunk;
+  return kVariants[var].kind != 0 ? kVariants[var].chunk - 8 : kVariants[var].chunk;
 }
 inline int variant_block_rows(int var)
 {
@@ -652,7 +884,7 @@ inline int variant_block_rows(int var)
 // Launches the SpMV kernel of variant `var` over `nblocks` row blocks (built
 // with variant_block_nnz(var) entries per block); returns the number of
 // partial sums written (MODE != kPlain).
-int pipe_grid(const void* kernel, int64_t nblocks);
+int pipe_grid(const void* kernel, int64_t nblocks, int threads = kBS);
 
 template <typename V, typename X, typename Acc, int MODE>
 int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t* rowptr,
@@ -663,7 +895,7 @@ int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t
   if (nblocks <= 0)
     return 0;
   int64_t grid = nblocks;
-#define SPMVX_CASE(i, K, C, VV, U, N, B)                                                     \
+#define SPMVX_CASE(i, K, C, VV, U, N, B, O)                                                  \
   case i:                                                                                    \
     if constexpr (kIsComplex<V> && (K != 0 || C > 2048))                                     \
       throw Error(SPMVX_ERR_ARG, "kernel variant not available for complex values");         \
@@ -671,6 +903,15 @@ int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t
       csr_stream<V, X, Acc, MODE, C, VV, U, N, B><<<dim3(static_cast<unsigned>(grid)),       \
                                                     dim3(B), 0, s>>>(                       \
           rowptr, colidx, val, x, y, blocks, nblocks, b, partials, st, it);                  \
+    else if constexpr (K == 2)                                                               \
+    {                                                                                        \
+      grid = pipe_grid(                                                                      \
+          reinterpret_cast<const void*>(&csr_defer<V, X, Acc, MODE, C, VV, N, B, O>),           \
+          nblocks, B);                                                                       \
+      csr_defer<V, X, Acc, MODE, C, VV, N, B, O><<<dim3(static_cast<unsigned>(grid)),        \
+                                                      dim3(B), 0, s>>>(                      \
+          rowptr, colidx, val, x, y, blocks, nblocks, b, partials, st, it);                  \
+    }                                                                                        \
     else                                                                                     \
     {                                                                                        \
       grid = pipe_grid(reinterpret_cast<const void*>(&csr_pipe<V, X, Acc, MODE, C, VV, N>),  \

@@ -168,7 +168,7 @@ const SliceLayout& Pattern::slice_layout(int vw)
   return ref;
 }
 
-int pipe_grid(const void* kernel, int64_t nblocks)
+int pipe_grid(const void* kernel, int64_t nblocks, int threads)
 {
   // all workgroups resident at once: CUs x the kernel's occupancy
   static int ncu = [] {
@@ -183,7 +183,7 @@ int pipe_grid(const void* kernel, int64_t nblocks)
   if (it == occ.end())
   {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBS, 0) != hipSuccess
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess
         || per_cu <= 0)
       per_cu = 4;
     it = occ.emplace(kernel, per_cu).first;
@@ -678,7 +678,8 @@ int spmvx_matrix_get_kernel(spmvx_matrix A, int32_t* variant, char* name, int32_
       else
       {
         const SpmvVariant& k = kVariants[v];
-        d = std::string(k.kind == 0 ? "csr_stream" : "csr_pipe") + " (" + std::to_string(k.bs)
+        d = std::string(k.kind == 0 ? "csr_stream" : k.kind == 1 ? "csr_pipe" : "csr_defer")
+            + " (" + std::to_string(k.bs)
             + " threads, chunk " + std::to_string(k.chunk) + ", vec " + std::to_string(k.vec)
             + ", unroll " + std::to_string(k.unroll) + (k.nt ? ", non-temporal" : "") + ")";
       }
