@@ -182,6 +182,16 @@ int spmvx_matrix_arrays(spmvx_matrix A, const int32_t** rowptr_dev,
 #define SPMVX_SLICED_VARIANT 15
 int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant);
 int spmvx_num_kernels(void);
+/* How rows longer than a kernel's staging block (2048-8192 entries, per
+ * variant) are summed.  0 (default): in stored order by one lane -- the
+ * sequential sum of Matrix.cpp:203, bit-identical, one dependent add per
+ * entry.  1: by the whole workgroup, each thread adding a strided subset in
+ * order, then a fixed-order tree -- deterministic, rounded differently from
+ * the sequential sum (relative to sum |a_ij x_j| the difference is a few
+ * ulps), and much faster for rows of 10^4 entries and more.  Real values only.
+ * Applies to products, transposed products, CG and refinement with A; copies
+ * made by spmvx_matrix_create_like inherit it.  No reference counterpart. */
+int spmvx_matrix_set_long_rows(spmvx_matrix A, int32_t parallel);
 /* The variant A uses and a short description of its kernel (written into
  * name[0:name_len), NUL-terminated; name may be NULL). */
 int spmvx_matrix_get_kernel(spmvx_matrix A, int32_t* variant, char* name, int32_t name_len);

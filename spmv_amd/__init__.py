@@ -97,6 +97,7 @@ def lib():
         "spmvx_matrix_arrays": ([P, P, P, P], ctypes.c_int),
         "spmvx_matrix_set_values": ([P, P, P], ctypes.c_int),
         "spmvx_matrix_set_kernel": ([P, i32], ctypes.c_int),
+        "spmvx_matrix_set_long_rows": ([P, i32], ctypes.c_int),
         "spmvx_num_kernels": ([], ctypes.c_int),
         "spmvx_matrix_sliced_info": ([P, P, P, P], ctypes.c_int),
         "spmvx_matrix_get_kernel": ([P, P, ctypes.c_char_p, i32], ctypes.c_int),
@@ -383,6 +384,13 @@ class Matrix:
         ``SLICED_VARIANT`` selects the sliced row-pattern layout and raises
         SpmvError with the reason when the matrix does not fit it."""
         _check(lib().spmvx_matrix_set_kernel(self._h, int(variant)))
+
+    def set_long_rows(self, parallel):
+        """Rows longer than a kernel's staging block: summed in stored order by
+        one lane (False, the default: bit-identical to the sequential product)
+        or by the whole workgroup in a fixed tree (True: deterministic, rounded
+        differently, much faster for very long rows).  Real values only."""
+        _check(lib().spmvx_matrix_set_long_rows(self._h, 1 if parallel else 0))
 
     def kernel(self):
         """(variant, description) of the SpMV kernel this matrix uses."""

@@ -116,7 +116,7 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
                : launch_csr_stream<V, X, Acc, kDot>(
                      A->variant, rb.nblocks, qq, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
                      static_cast<const V*>(vals), p, y, rb.desc.as<int2>(), nullptr, part_s,
-                     st, 1);
+                     st, 1, A->long_rows);
     cg_alpha_finalize<Acc><<<1, kFinBS, 0, qq>>>(part_s, nparts, st, 1);
     cg_update_r<X, Acc><<<vg, kVecBS, 0, qq>>>(n, r, y, part_1, st, 1);
     cg_beta_finalize<Acc><<<1, kFinBS, 0, qq>>>(part_1, vg, st, hist_dev, 1);
@@ -126,7 +126,8 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
   const void* halo = col_map ? col_map->h_halo.ptr : nullptr;
   const int32_t halo_dt = col_map ? col_map->halo_dtype : -2;
   if (!w.exec || w.g_x != x || w.g_hist != hist_dev || w.g_map != col_map || w.g_halo != halo
-      || w.g_halo_dtype != halo_dt || w.g_variant != A->variant || w.g_mem != w.mem.ptr
+      || w.g_halo_dtype != halo_dt || w.g_variant != A->variant
+      || w.g_long_rows != A->long_rows || w.g_mem != w.mem.ptr
       || w.g_values != vals)
   {
     if (w.exec)
@@ -150,6 +151,7 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
     w.g_halo = halo;
     w.g_halo_dtype = halo_dt;
     w.g_variant = A->variant;
+    w.g_long_rows = A->long_rows;
     w.g_mem = w.mem.ptr;
     w.g_values = vals;
   }
@@ -324,7 +326,7 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
               : launch_csr_stream<double, double, double, kResid>(
                     A64->variant, rb64.nblocks, s, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
                     static_cast<const double*>(A64->values.ptr), x, r, rb64.desc.as<int2>(), b,
-                    static_cast<double*>(w.red_spmv), st, 0);
+                    static_cast<double*>(w.red_spmv), st, 0, A64->long_rows);
       sum_finalize<double><<<1, kFinBS, 0, s>>>(static_cast<double*>(w.red_spmv), nparts, st);
       check_launch("csr_stream<kResid>");
       SPMVX_HIP(hipMemcpyAsync(flag, st, sizeof(CgState<double>), hipMemcpyDeviceToHost, s));

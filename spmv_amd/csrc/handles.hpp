@@ -188,6 +188,7 @@ struct CgWorkspace
   const void* g_halo = nullptr;
   int32_t g_halo_dtype = -2;
   int32_t g_variant = -1;
+  int32_t g_long_rows = -1;
   const void* g_mem = nullptr;
   const void* g_values = nullptr;
   DevBuf mem;
@@ -211,7 +212,8 @@ struct CgWorkspace
 struct spmvx_matrix_s
 {
   int32_t dtype = SPMVX_F64;
-  int32_t variant = 0; // SpMV kernel variant (kVariants)
+  int32_t variant = 0;   // SpMV kernel variant (kVariants)
+  int32_t long_rows = 0; // 1: rows longer than a block are summed in parallel
   std::shared_ptr<spmvx::Pattern> pat;
   spmvx::DevBuf values;
   // transposed copy for spmv_t (built on first use)

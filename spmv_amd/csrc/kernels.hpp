@@ -247,6 +247,73 @@ __device__ __forceinline__ Acc row_sum(const Acc* prod, int a, int e)
   return s;
 }
 
+// Sum of a row longer than a block's staging chunk (entries [k0, k1)),
+// valid in thread 0.  Ordered (par == 0): the rounded products are staged
+// CHUNK at a time in LDS and thread 0 adds them in stored order, starting from
+// 0 -- the sequential product, bit-identical to the reference; its cost is
+// one dependent add per entry in one lane.  Parallel (par != 0, real values):
+// thread t adds the products of entries k0 + t, k0 + t + BS, ... in that
+// order and the workgroup's partial sums are combined in a fixed tree
+// (block_sum): deterministic, but rounded differently from the sequential
+// sum.  Called by all threads; starts after and ends with a barrier.
+template <typename V, typename X, typename Acc, int CHUNK, int BS, bool NT>
+__device__ __forceinline__ Acc long_row_sum(const int32_t* __restrict__ colidx,
+                                            const V* __restrict__ val, const X* __restrict__ x,
+                                            int64_t k0, int64_t k1, Acc* prod, Acc* red,
+                                            int32_t par)
+{
+  const int tid = threadIdx.x;
+  if constexpr (!kIsComplex<Acc>)
+  {
+    if (par)
+    {
+      // eight entries' loads in flight per thread; the adds stay in order
+      constexpr int U = 8;
+      Acc s = Acc(0);
+      int64_t k = k0 + tid;
+      for (; k + (U - 1) * BS < k1; k += U * BS)
+      {
+        int32_t c[U];
+        Acc a[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+        {
+          c[j] = ld_stream<NT>(colidx + k + j * BS);
+          a[j] = static_cast<Acc>(ld_val<NT>(val, k + j * BS));
+        }
+        X xv[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+          xv[j] = x[c[j]];
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+          s += a[j] * static_cast<Acc>(xv[j]);
+      }
+      for (; k < k1; k += BS)
+        s += static_cast<Acc>(ld_val<NT>(val, k)) * static_cast<Acc>(x[ld_stream<NT>(colidx + k)]);
+      s = block_sum<Acc, true>(s, red);
+      lds_barrier(); // red is reused by the caller
+      return s;
+    }
+  }
+  Acc s = Acc(0);
+  for (int64_t c0 = k0; c0 < k1; c0 += CHUNK)
+  {
+    const int n = static_cast<int>((k1 - c0) < CHUNK ? (k1 - c0) : CHUNK);
+    for (int t = tid; t < n; t += BS)
+    {
+      const int64_t k = c0 + t;
+      prod[t] = static_cast<Acc>(ld_val<NT>(val, k)) * static_cast<Acc>(x[colidx[k]]);
+    }
+    lds_barrier();
+    if (tid == 0)
+      for (int q = 0; q < n; ++q)
+        s += prod[q];
+    lds_barrier();
+  }
+  return s;
+}
+
 // Kernel configuration: products staged per block (CHUNK), consecutive
 // entries each thread loads with one vector load per array (VEC; the entry
 // range of a block is widened down to a multiple of VEC, the arrays are
@@ -261,7 +328,7 @@ __global__ __launch_bounds__(BS) void csr_stream(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
     const int2* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
-    Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it)
+    Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it, int32_t long_par)
 {
   if (MODE != kPlain && !cg_live(st, it))
     return;
@@ -348,23 +415,9 @@ __global__ __launch_bounds__(BS) void csr_stream(
   }
   else
   {
-    // One row longer than CHUNK: staged chunk by chunk, summed in order by
-    // thread 0.
-    Acc s = Acc(0);
-    for (int64_t c0 = k0; c0 < k1; c0 += CHUNK)
-    {
-      const int n = static_cast<int>((k1 - c0) < CHUNK ? (k1 - c0) : CHUNK);
-      lds_barrier();
-      for (int t = tid; t < n; t += BS)
-      {
-        const int64_t k = c0 + t;
-        prod[t] = static_cast<Acc>(ld_val<NT>(val, k)) * static_cast<Acc>(x[colidx[k]]);
-      }
-      lds_barrier();
-      if (tid == 0)
-        for (int q = 0; q < n; ++q)
-          s += prod[q];
-    }
+    // one row longer than CHUNK
+    const Acc s =
+        long_row_sum<V, X, Acc, CHUNK, BS, NT>(colidx, val, x, k0, k1, prod, red, long_par);
     if (tid == 0)
     {
       if (MODE == kResid)
@@ -460,7 +513,8 @@ __device__ __forceinline__ void pipe_compute(const PipeStage<V, VEC, PER>& S,
                                              const V* __restrict__ val,
                                              const X* __restrict__ x, X* __restrict__ y,
                                              const X* __restrict__ b, int2 d0, int2 d1,
-                                             Acc* prod, Acc& part, int long_limit)
+                                             Acc* prod, Acc* red, Acc& part, int long_limit,
+                                             int32_t long_par)
 {
   const int tid = threadIdx.x;
   const int32_t r0 = d0.x, r1 = d1.x, k0 = d0.y, k1 = d1.y;
@@ -503,22 +557,9 @@ __device__ __forceinline__ void pipe_compute(const PipeStage<V, VEC, PER>& S,
   }
   else
   {
-    // one long row, staged chunk by chunk and summed in order by thread 0
-    Acc s = Acc(0);
-    for (int64_t c0 = k0; c0 < k1; c0 += CHUNK)
-    {
-      const int n = static_cast<int>((k1 - c0) < CHUNK ? (k1 - c0) : CHUNK);
-      for (int t = tid; t < n; t += kBS)
-      {
-        const int64_t k = c0 + t;
-        prod[t] = static_cast<Acc>(ld_val<NT>(val, k)) * static_cast<Acc>(x[colidx[k]]);
-      }
-      lds_barrier();
-      if (tid == 0)
-        for (int q = 0; q < n; ++q)
-          s += prod[q];
-      lds_barrier();
-    }
+    // one long row
+    const Acc s =
+        long_row_sum<V, X, Acc, CHUNK, kBS, NT>(colidx, val, x, k0, k1, prod, red, long_par);
     if (tid == 0)
     {
       if (MODE == kResid)
@@ -543,7 +584,7 @@ __global__ __launch_bounds__(kBS) void csr_pipe(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
     const int2* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
-    Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it)
+    Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it, int32_t long_par)
 {
   if (MODE != kPlain && !cg_live(st, it))
     return;
@@ -572,7 +613,7 @@ __global__ __launch_bounds__(kBS) void csr_pipe(
       pipe_issue<NT>(nxt, rowptr, colidx, val, more ? a1 : a0, more ? a2 : a1, kLong);
       const int2 a3 = blocks[bi + 3 < be ? bi + 3 : be];
       pipe_compute<V, X, Acc, MODE, CHUNK, VEC, PER, NT>(cur, xg, colidx, val, x, y, b, a0, a1,
-                                                          prod, part, kLong);
+                                                          prod, red, part, kLong, long_par);
       a0 = a1;
       a1 = a2;
       a2 = a3;
@@ -672,8 +713,8 @@ __device__ __forceinline__ void defer_compute(const PipeStage<V, VEC, PER>& S,
                                               const int32_t* __restrict__ colidx,
                                               const V* __restrict__ val,
                                               const X* __restrict__ x, X* __restrict__ y,
-                                              X aux, int2 d0, int2 d1, Acc* prod,
-                                              DeferBuf<X>& D, Acc& part)
+                                              X aux, int2 d0, int2 d1, Acc* prod, Acc* red,
+                                              DeferBuf<X>& D, Acc& part, int32_t long_par)
 {
   // aux: x[row] (kDot) or b[row] (kResid) of this thread's row, loaded by the
   // caller before the gathers (a load issued here would wait for the next
@@ -726,22 +767,9 @@ __device__ __forceinline__ void defer_compute(const PipeStage<V, VEC, PER>& S,
   }
   else
   {
-    // one long row, staged chunk by chunk and summed in order by thread 0
-    Acc s = Acc(0);
-    for (int64_t c0 = k0; c0 < k1; c0 += CHUNK)
-    {
-      const int n = static_cast<int>((k1 - c0) < CHUNK ? (k1 - c0) : CHUNK);
-      for (int t = tid; t < n; t += BS)
-      {
-        const int64_t k = c0 + t;
-        prod[t] = static_cast<Acc>(ld_val<NT>(val, k)) * static_cast<Acc>(x[colidx[k]]);
-      }
-      lds_barrier();
-      if (tid == 0)
-        for (int q = 0; q < n; ++q)
-          s += prod[q];
-      lds_barrier();
-    }
+    // one long row
+    const Acc s =
+        long_row_sum<V, X, Acc, CHUNK, BS, NT>(colidx, val, x, k0, k1, prod, red, long_par);
     if (tid == 0) // row r0 is thread 0's row
     {
       const X yi = MODE == kResid ? aux - static_cast<X>(s) : static_cast<X>(s);
@@ -761,7 +789,7 @@ __global__ __launch_bounds__(BS, OCC* BS / 256) void csr_defer( // OCC workgroup
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
     const int2* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
-    Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it)
+    Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it, int32_t long_par)
 {
   if (MODE != kPlain && !cg_live(st, it))
     return;
@@ -810,8 +838,8 @@ __global__ __launch_bounds__(BS, OCC* BS / 256) void csr_defer( // OCC workgroup
                                       more ? n1 : c1, kLong);
       const int64_t ab = bi + 2 * stride < end ? bi + 2 * stride : bi;
       const int2 a0 = blocks[ab], a1 = blocks[ab + 1];
-      defer_compute<V, X, Acc, MODE, CHUNK, VEC, PER, OCC, BS, NT>(cur, xg, colidx, val, x, y,
-                                                                   aux, c0, c1, prod, D, part);
+      defer_compute<V, X, Acc, MODE, CHUNK, VEC, PER, OCC, BS, NT>(
+          cur, xg, colidx, val, x, y, aux, c0, c1, prod, red, D, part, long_par);
       c0 = n0;
       c1 = n1;
       n0 = a0;
@@ -883,14 +911,15 @@ inline int variant_block_rows(int var)
 
 // Launches the SpMV kernel of variant `var` over `nblocks` row blocks (built
 // with variant_block_nnz(var) entries per block); returns the number of
-// partial sums written (MODE != kPlain).
+// partial sums written (MODE != kPlain).  long_par: rows longer than a block
+// are summed in parallel (long_row_sum) instead of in stored order.
 int pipe_grid(const void* kernel, int64_t nblocks, int threads = kBS);
 
 template <typename V, typename X, typename Acc, int MODE>
 int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t* rowptr,
                           const int32_t* colidx, const V* val, const X* x, X* y,
                           const int2* blocks, const X* b, Acc* partials,
-                          const CgState<Acc>* st, int32_t it)
+                          const CgState<Acc>* st, int32_t it, int32_t long_par = 0)
 {
   if (nblocks <= 0)
     return 0;
@@ -902,7 +931,7 @@ int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t
     else if constexpr (K == 0)                                                               \
       csr_stream<V, X, Acc, MODE, C, VV, U, N, B><<<dim3(static_cast<unsigned>(grid)),       \
                                                     dim3(B), 0, s>>>(                       \
-          rowptr, colidx, val, x, y, blocks, nblocks, b, partials, st, it);                  \
+          rowptr, colidx, val, x, y, blocks, nblocks, b, partials, st, it, long_par);        \
     else if constexpr (K == 2)                                                               \
     {                                                                                        \
       grid = pipe_grid(                                                                      \
@@ -910,7 +939,7 @@ int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t
           nblocks, B);                                                                       \
       csr_defer<V, X, Acc, MODE, C, VV, N, B, O><<<dim3(static_cast<unsigned>(grid)),        \
                                                       dim3(B), 0, s>>>(                      \
-          rowptr, colidx, val, x, y, blocks, nblocks, b, partials, st, it);                  \
+          rowptr, colidx, val, x, y, blocks, nblocks, b, partials, st, it, long_par);        \
     }                                                                                        \
     else                                                                                     \
     {                                                                                        \
@@ -918,7 +947,7 @@ int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t
                        nblocks);                                                             \
       csr_pipe<V, X, Acc, MODE, C, VV, N><<<dim3(static_cast<unsigned>(grid)), dim3(kBS), 0, \
                                             s>>>(rowptr, colidx, val, x, y, blocks, nblocks, \
-                                                 b, partials, st, it);                       \
+                                                 b, partials, st, it, long_par);             \
     }                                                                                        \
     break;
   switch (var)

@@ -350,29 +350,31 @@ __global__ __launch_bounds__(kBS) void sddmm_kernel(const int32_t* __restrict__ 
 }
 
 template <typename V, typename X, typename Acc>
-void launch_plain(int var, Pattern& P, const void* vals, const void* x, void* y, hipStream_t s)
+void launch_plain(int var, Pattern& P, const void* vals, const void* x, void* y, hipStream_t s,
+                  int32_t long_par)
 {
   const RowBlocks& rb = variant_blocks(P, var);
   launch_csr_stream<V, X, Acc, kPlain>(var, rb.nblocks, s, P.rowptr.as<int32_t>(),
                                        P.colidx.as<int32_t>(), static_cast<const V*>(vals),
                                        static_cast<const X*>(x), static_cast<X*>(y),
-                                       rb.desc.as<int2>(), nullptr, nullptr, nullptr, 0);
+                                       rb.desc.as<int2>(), nullptr, nullptr, nullptr, 0,
+                                       long_par);
   check_launch("csr SpMV");
 }
 
 void launch_spmv(int32_t dtype, int var, Pattern& P, const void* vals, const void* x, void* y,
-                 hipStream_t s)
+                 hipStream_t s, int32_t long_par)
 {
   if (dtype == SPMVX_F64)
-    launch_plain<double, double, double>(var, P, vals, x, y, s);
+    launch_plain<double, double, double>(var, P, vals, x, y, s, long_par);
   else if (dtype == SPMVX_F32)
-    launch_plain<float, float, float>(var, P, vals, x, y, s);
+    launch_plain<float, float, float>(var, P, vals, x, y, s, long_par);
   else if (dtype == SPMVX_BF16)
-    launch_plain<bf16_t, float, float>(var, P, vals, x, y, s);
+    launch_plain<bf16_t, float, float>(var, P, vals, x, y, s, long_par);
   else if (dtype == SPMVX_C64)
-    launch_plain<cplx<double>, cplx<double>, cplx<double>>(var, P, vals, x, y, s);
+    launch_plain<cplx<double>, cplx<double>, cplx<double>>(var, P, vals, x, y, s, 0);
   else
-    launch_plain<cplx<float>, cplx<float>, cplx<float>>(var, P, vals, x, y, s);
+    launch_plain<cplx<float>, cplx<float>, cplx<float>>(var, P, vals, x, y, s, 0);
 }
 
 // Fills A->svalues from A->values on stream s (layout built beforehand).
@@ -510,7 +512,7 @@ void launch_matrix_spmv(spmvx_matrix_s* A, const void* x, void* y, hipStream_t s
 {
   if (A->variant != kSlicedVariant)
   {
-    launch_spmv(A->dtype, A->variant, *A->pat, A->values.ptr, x, y, s);
+    launch_spmv(A->dtype, A->variant, *A->pat, A->values.ptr, x, y, s, A->long_rows);
     return;
   }
   const SlicedView v = sliced_view(A);
@@ -585,6 +587,7 @@ int spmvx_matrix_create_like(spmvx_matrix base, int32_t dtype, spmvx_matrix* out
     auto A = std::make_unique<spmvx_matrix_s>();
     A->dtype = dtype;
     A->variant = default_variant(dtype);
+    A->long_rows = base->long_rows;
     A->pat = base->pat;
     const int64_t nnz = base->pat->nnz;
     A->values.ensure(es * static_cast<size_t>(nnz + 16));
@@ -663,6 +666,17 @@ int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant)
 
 int spmvx_num_kernels(void) { return kSlicedVariant + 1; }
 
+int spmvx_matrix_set_long_rows(spmvx_matrix A, int32_t parallel)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    SPMVX_REQUIRE(parallel == 0 || parallel == 1, "long-row mode must be 0 or 1");
+    SPMVX_REQUIRE(parallel == 0 || !is_complex(A->dtype),
+                  "parallel long rows are for real values");
+    A->long_rows = parallel;
+  });
+}
+
 int spmvx_matrix_get_kernel(spmvx_matrix A, int32_t* variant, char* name, int32_t name_len)
 {
   return guarded([&] {
@@ -722,7 +736,8 @@ int spmvx_spmv_t(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
     if (!A->tpat)
       build_transpose(A);
     SPMVX_REQUIRE(A->pat->ncols == 0 || y != nullptr, "y is NULL");
-    launch_spmv(A->dtype, csr_variant(A), *A->tpat, A->tvalues.ptr, x, y, as_stream(stream));
+    launch_spmv(A->dtype, csr_variant(A), *A->tpat, A->tvalues.ptr, x, y, as_stream(stream),
+                A->long_rows);
   });
 }
 

@@ -204,6 +204,143 @@ def test_all_kernel_variants_bitexact(name, A):
         assert np.array_equal(yb, ref_b), v
 
 
+# ---------------------------------------------------------------- rows longer than a block
+def very_long_rows(n=40000, seed=12):
+    """Every row holds column 0 (so column 0 of A, row 0 of A^T, is dense);
+    row 0 is dense; rows 1, 7 and 4000 hold 30000, 12000 and 8500 entries
+    (past every variant's 2048-8192-entry block); the rest 1-30 entries."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        if i == 0:
+            c = np.arange(n)
+        else:
+            k = {1: 30000, 7: 12000, 4000: 8500}.get(i, int(rng.integers(1, 30)))
+            c = np.union1d([0], rng.choice(np.arange(1, n), k, replace=False))
+        rows.append(c.astype(np.int32))
+    rowptr = np.zeros(n + 1, np.int64)
+    rowptr[1:] = np.cumsum([len(r) for r in rows])
+    colidx = np.concatenate(rows)
+    val = P.uniform(seed, len(colidx), -1, 1)
+    return P.CSR(n, n, rowptr.astype(np.int32), colidx, val)
+
+
+def _exact_rows(A, val, x, dtype):
+    """Per row: the exact sum (math.fsum) of the products rounded in `dtype`,
+    and sum |a_ij x_j|."""
+    import math
+    prod = (val.astype(dtype) * x.astype(dtype)[A.colidx]).astype(np.float64)
+    ex = np.array([math.fsum(prod[A.rowptr[i]:A.rowptr[i + 1]]) for i in range(A.nrows)])
+    mag = np.add.reduceat(np.abs(prod), A.rowptr[:-1]) * (np.diff(A.rowptr) > 0)
+    return ex, mag
+
+
+def _check_long_rows(y, ex, mag, lens, u):
+    """Rows past every block (> 8192 entries) are summed by a workgroup of at
+    least 256 threads: each thread adds at most ceil(len/256) products, then
+    a tree of depth <= 10 (plus the final round) combines them.  Recursive
+    summation of m terms errs by at most m*u*sum|terms| (to first order)."""
+    long = lens > 8192
+    assert long.any()
+    m = -(-lens[long] // 256) + 12
+    assert np.all(np.abs(y[long] - ex[long]) <= 1.01 * m * u * mag[long])
+
+
+LONG = very_long_rows()
+
+
+def test_long_rows_ordered_all_variants_bitexact():
+    A = LONG
+    x = P.uniform(3, A.ncols, -1, 1)
+    want = O.spmv(_ocsr(A), x)
+    want_t = O.spmv_t(_ocsr(A), x)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    for v in range(S.SLICED_VARIANT):
+        M.set_kernel(v)
+        assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), want), v
+    assert np.array_equal(M.transpmult(_dev(x)).cpu().numpy(), want_t)
+
+
+def test_long_rows_parallel_all_variants():
+    """Parallel long rows: deterministic; rows that fit a block unchanged
+    (bit-exact); long rows within the summation bound of the exact sum."""
+    A = LONG
+    lens = np.diff(A.rowptr)
+    x = P.uniform(3, A.ncols, -1, 1)
+    want = O.spmv(_ocsr(A), x)
+    ex, mag = _exact_rows(A, A.val, x, np.float64)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    M.set_long_rows(True)
+    u = 2.0 ** -53
+    for v in range(S.SLICED_VARIANT):
+        M.set_kernel(v)
+        y = M.mult(_dev(x)).cpu().numpy()
+        assert np.array_equal(y, M.mult(_dev(x)).cpu().numpy()), v   # deterministic
+        short = lens <= 2040
+        assert np.array_equal(y[short], want[short]), v
+        _check_long_rows(y, ex, mag, lens, u)
+        # the mode is in effect: some long row is rounded differently
+        assert not np.array_equal(y[lens > 8192], want[lens > 8192]), v
+    # transposed product: row 0 of A^T holds all 40000 rows
+    import scipy.sparse as sp
+    T = sp.csr_matrix((A.val, A.colidx, A.rowptr), shape=(A.nrows, A.ncols)).T.tocsr()
+    At = P.CSR(A.ncols, A.nrows, T.indptr.astype(np.int32), T.indices.astype(np.int32), T.data)
+    ext, magt = _exact_rows(At, At.val, x, np.float64)
+    yt = M.transpmult(_dev(x)).cpu().numpy()
+    lt = np.diff(At.rowptr)
+    assert np.array_equal(yt[lt <= 2040], O.spmv_t(_ocsr(A), x)[lt <= 2040])
+    _check_long_rows(yt, ext, magt, lt, u)
+    # back to the ordered sum: bit-exact again
+    M.set_long_rows(False)
+    M.set_kernel(-1)
+    assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), want)
+
+
+def test_long_rows_parallel_f32_and_restrictions():
+    A = LONG
+    lens = np.diff(A.rowptr)
+    x = P.uniform(4, A.ncols, -1, 1).astype(np.float32)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    M.set_long_rows(True)
+    Mf = M.like("f32")      # inherits the mode
+    y = Mf.mult(_dev(x, torch.float32)).cpu().numpy().astype(np.float64)
+    ex, mag = _exact_rows(A, A.val.astype(np.float32), x, np.float32)
+    _check_long_rows(y, ex, mag, lens, 2.0 ** -24)
+    Mc = S.Matrix(A.rowptr, A.colidx, A.val.astype(np.complex128), ncols=A.ncols, dtype="c64")
+    with pytest.raises(S.SpmvError, match="real values"):
+        Mc.set_long_rows(True)
+    with pytest.raises(S.SpmvError, match="0 or 1"):
+        S._check(S.lib().spmvx_matrix_set_long_rows(M._h, 2))
+
+
+def test_long_rows_cg_both_modes():
+    """CG on an SPD arrow matrix whose first row and column are dense (20000
+    entries): both modes converge to the direct solution."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    n = 20000
+    rng = np.random.default_rng(5)
+    u = rng.uniform(-1, 0, n - 1) / n
+    off = sp.coo_matrix((np.concatenate([u, u]),
+                         (np.concatenate([np.zeros(n - 1, int), np.arange(1, n)]),
+                          np.concatenate([np.arange(1, n), np.zeros(n - 1, int)]))), shape=(n, n))
+    T = sp.diags([-0.3 * np.ones(n - 1), -0.3 * np.ones(n - 1)], [-1, 1])
+    Asp = (off + T).tocsr()
+    Asp = (Asp + sp.diags(1.1 * np.asarray(abs(Asp).sum(axis=1)).ravel() + 1.0)).tocsr()
+    Asp.sort_indices()
+    b = P.uniform(6, n, -1, 1)
+    xd = spla.spsolve(Asp.tocsc(), b)
+    its = []
+    for par in (False, True):
+        M = S.Matrix(Asp.indptr.astype(np.int32), Asp.indices.astype(np.int32), Asp.data, ncols=n)
+        M.set_long_rows(par)
+        x, k = S.cg(M, _dev(b), 500, 1e-12)
+        x = x.cpu().numpy()[:n]
+        assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+        its.append(k)
+    assert abs(its[0] - its[1]) <= 2
+
+
 # ---------------------------------------------------------------- sliced row-pattern layout
 SLICED_FIT = ["demo1d", "poisson5", "poisson7", "stencil27_ghost", "diffusion", "one_row"]
 
