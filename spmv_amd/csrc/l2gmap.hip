@@ -89,7 +89,7 @@ void l2gmap_prepare_device(spmvx_l2gmap_s* m)
 {
   if (m->device_ready)
     return;
-  const size_t nidx = m->indexbuf.size(), ng = m->ghosts.size();
+  const size_t nidx = m->indexbuf.size();
   m->d_indexbuf.ensure(4 * std::max<size_t>(nidx, 1));
   if (nidx)
     SPMVX_HIP(hipMemcpy(m->d_indexbuf.ptr, m->indexbuf.data(), 4 * nidx, hipMemcpyHostToDevice));
@@ -114,7 +114,7 @@ void l2gmap_update_async(spmvx_l2gmap_s* m, int32_t dtype, void* vec, hipStream_
   const int64_t ng = static_cast<int64_t>(m->ghosts.size());
   if (nidx == 0 && ng == 0)
     return;
-  const size_t es = vec_elem(dtype);
+  (void)vec_elem(dtype); // rejects an unknown dtype
   // receive: the unpack kernel reads the halo (what the neighbours send)
   // straight from its pinned host buffer.  No separate host-to-device copy:
   // it would be one more operation per update, and HIP enqueues a pinned
