@@ -305,20 +305,21 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
 #pragma unroll
       for (int j = 0; j < NRHS; ++j)
         alpha[j] = active[j] ? rr.v[j] / pdy.v[j] : 0.0f;
+      // r -= alpha y and ||r||^2; x += alpha p is applied in the pass that
+      // forms the next direction (it reads p there anyway): one vector pass
+      // fewer per iteration, the same operations on every element
       VecN<NRHS> rn{};
       for (int32_t i = lo + static_cast<int32_t>(threadIdx.x); i < hi; i += blockDim.x)
       {
         const int64_t o = static_cast<int64_t>(i) * NRHS;
-        VecN<NRHS> ri = ldn<NRHS>(r + o), xi = ldn<NRHS>(x + o);
-        const VecN<NRHS> yi = ldn<NRHS>(y + o), pi = ldn<NRHS>(p + o);
+        VecN<NRHS> ri = ldn<NRHS>(r + o);
+        const VecN<NRHS> yi = ldn<NRHS>(y + o);
 #pragma unroll
         for (int j = 0; j < NRHS; ++j)
         {
-          xi.v[j] = xi.v[j] + alpha[j] * pi.v[j];
           ri.v[j] = ri.v[j] - alpha[j] * yi.v[j];
           rn.v[j] += ri.v[j] * ri.v[j];
         }
-        stn<NRHS>(x + o, xi);
         stn<NRHS>(r + o, ri);
       }
       rn = team_sum<NRHS>(rn, tred, 0, W, m, bar, nbar, err, red);
@@ -338,15 +339,32 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
         any = any || active[j];
       }
       if (!any)
+      {
+        // last iteration: x += alpha p only
+        for (int32_t i = lo + static_cast<int32_t>(threadIdx.x); i < hi; i += blockDim.x)
+        {
+          const int64_t o = static_cast<int64_t>(i) * NRHS;
+          VecN<NRHS> xi = ldn<NRHS>(x + o);
+          const VecN<NRHS> pi = ldn<NRHS>(p + o);
+#pragma unroll
+          for (int j = 0; j < NRHS; ++j)
+            xi.v[j] = xi.v[j] + alpha[j] * pi.v[j];
+          stn<NRHS>(x + o, xi);
+        }
         break;
+      }
       for (int32_t i = lo + static_cast<int32_t>(threadIdx.x); i < hi; i += blockDim.x)
       {
         const int64_t o = static_cast<int64_t>(i) * NRHS;
         const VecN<NRHS> ri = ldn<NRHS>(r + o);
-        VecN<NRHS> pi = ldn<NRHS>(p + o);
+        VecN<NRHS> pi = ldn<NRHS>(p + o), xi = ldn<NRHS>(x + o);
 #pragma unroll
         for (int j = 0; j < NRHS; ++j)
+        {
+          xi.v[j] = xi.v[j] + alpha[j] * pi.v[j];
           pi.v[j] = pi.v[j] * beta[j] + ri.v[j];
+        }
+        stn<NRHS>(x + o, xi);
         stn<NRHS>(p + o, pi);
       }
       if (W > 1)
