@@ -114,10 +114,11 @@ int spmvx_l2gmap_global_to_local(spmvx_l2gmap map, int64_t global_index, int32_t
 /*
  * Single-process transport.  What the neighbours would send into this rank's
  * ghost region comes from a host "halo" buffer of num_ghosts values of
- * `dtype` (F64 or F32), copied into library-owned pinned memory; NULL means
- * zeros (the default).  The unpack kernel of every later update reads that
- * memory directly, so call this only when no update is in flight
- * (synchronise the stream first).
+ * `dtype` (F64, F32, C64 or C32), copied into library-owned device memory
+ * (one synchronous host-to-device copy per call); NULL means zeros (the
+ * default).  Every later update unpacks that device copy, so call this only
+ * when no update is in flight (synchronise the stream first), and again
+ * whenever the neighbours' values change.
  */
 int spmvx_l2gmap_set_halo(spmvx_l2gmap map, int32_t dtype, const void* halo_host);
 

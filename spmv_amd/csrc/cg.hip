@@ -123,7 +123,7 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
     cg_update_xp<X, Acc><<<vg, kVecBS, 0, qq>>>(n, x, p, r, st, 1);
   };
 
-  const void* halo = col_map ? col_map->h_halo.ptr : nullptr;
+  const void* halo = col_map ? col_map->d_halo.ptr : nullptr;
   const int32_t halo_dt = col_map ? col_map->halo_dtype : -2;
   if (!w.exec || w.g_x != x || w.g_hist != hist_dev || w.g_map != col_map || w.g_halo != halo
       || w.g_halo_dtype != halo_dt || w.g_variant != A->variant

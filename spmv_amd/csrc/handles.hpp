@@ -240,9 +240,9 @@ struct spmvx_l2gmap_s
   int64_t local_size = 0;
   // device side (allocated on first use)
   spmvx::DevBuf d_indexbuf, d_send, d_rev_target, d_rev_ptr, d_rev_pos;
-  spmvx::PinnedBuf h_halo, h_send, h_in;
-  void* d_halo_view = nullptr; // device address of h_halo (read by the unpack kernel)
-  int32_t halo_dtype = -1; // dtype of the halo in h_halo, -1 = zeros
+  spmvx::PinnedBuf h_send, h_in;
+  spmvx::DevBuf d_halo;    // device copy of the halo (what the neighbours send)
+  int32_t halo_dtype = -1; // dtype of the halo in d_halo, -1 = zeros
   bool device_ready = false;
 };
 

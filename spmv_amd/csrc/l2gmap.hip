@@ -115,15 +115,13 @@ void l2gmap_update_async(spmvx_l2gmap_s* m, int32_t dtype, void* vec, hipStream_
   if (nidx == 0 && ng == 0)
     return;
   (void)vec_elem(dtype); // rejects an unknown dtype
-  // receive: the unpack kernel reads the halo (what the neighbours send)
-  // straight from its pinned host buffer.  No separate host-to-device copy:
-  // it would be one more operation per update, and HIP enqueues a pinned
-  // copy on a stream that waits for an event by blocking the host.
+  // receive: the unpack kernel reads the device copy of the halo (what the
+  // neighbours send) made by set_halo
   const void* inbox = nullptr;
   if (ng > 0 && m->halo_dtype >= 0)
   {
     SPMVX_REQUIRE(m->halo_dtype == dtype, "halo dtype differs from the vector dtype");
-    inbox = m->d_halo_view;
+    inbox = m->d_halo.ptr;
   }
   const int g = grid_for(std::max(nidx, ng));
   with_vec_type(dtype, [&](auto* tag) {
@@ -337,10 +335,9 @@ int spmvx_l2gmap_set_halo(spmvx_l2gmap m, int32_t dtype, const void* halo_host)
     }
     const size_t es = vec_elem(dtype);
     const size_t ng = m->ghosts.size();
-    m->h_halo.ensure(es * std::max<size_t>(ng, 1));
+    m->d_halo.ensure(es * std::max<size_t>(ng, 1));
     if (ng)
-      std::memcpy(m->h_halo.ptr, halo_host, es * ng);
-    SPMVX_HIP(hipHostGetDevicePointer(&m->d_halo_view, m->h_halo.ptr, 0));
+      SPMVX_HIP(hipMemcpy(m->d_halo.ptr, halo_host, es * ng, hipMemcpyHostToDevice));
     m->halo_dtype = dtype;
   });
 }
