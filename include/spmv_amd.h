@@ -24,7 +24,12 @@
  *     2*nnz doubles.  They support the products, transposed products, value
  *     updates and the L2GMap operations (kernel variants 0-4); CG, refinement,
  *     SDDMM, the sliced layout and create_like are for real values.
- *   - Handles are not thread safe; calls on one handle must be serialised.
+ *   - Handles are not thread safe; calls on one handle must be serialised,
+ *     and so must calls that use one L2GMap (a matrix's column map, a CG's
+ *     map) -- as in the reference, where update is collective.  Matrices
+ *     made by spmvx_matrix_create_like from one base share its pattern; the
+ *     library locks what it builds there, so they may be used from
+ *     different threads.
  */
 #ifndef SPMV_AMD_H
 #define SPMV_AMD_H

@@ -3,6 +3,7 @@
 
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -124,6 +125,10 @@ struct Pattern
   std::map<int, std::unique_ptr<SliceLayout>> slices; // by VW
   const RowPatterns* row_patterns();
   const SliceLayout& slice_layout(int vw);
+  // the lazily built parts above are shared by every matrix made with
+  // spmvx_matrix_create_like from one base, which callers may use from
+  // different threads
+  std::recursive_mutex lazy_mu;
 };
 
 // Row blocks for the ordered stream kernel: consecutive rows, at most

@@ -40,6 +40,7 @@ std::vector<int2> row_blocks(const int32_t* rowptr, int64_t nrows, int64_t max_n
 
 const RowBlocks& Pattern::blocks_for(int chunk, int rows)
 {
+  std::lock_guard<std::recursive_mutex> lock(lazy_mu);
   const std::pair<int, int> key(chunk, rows);
   auto it = blocks.find(key);
   if (it != blocks.end())
@@ -56,6 +57,7 @@ const RowBlocks& Pattern::blocks_for(int chunk, int rows)
 
 const RowPatterns* Pattern::row_patterns()
 {
+  std::lock_guard<std::recursive_mutex> lock(lazy_mu);
   if (rpat_tried)
     return rpat.get();
   rpat_tried = true;
@@ -140,6 +142,7 @@ const RowPatterns* Pattern::row_patterns()
 
 const SliceLayout& Pattern::slice_layout(int vw)
 {
+  std::lock_guard<std::recursive_mutex> lock(lazy_mu);
   auto it = slices.find(vw);
   if (it != slices.end())
     return *it->second;
@@ -179,6 +182,8 @@ int pipe_grid(const void* kernel, int64_t nblocks, int threads)
     return n > 0 ? n : 256;
   }();
   static std::map<const void*, int> occ;
+  static std::mutex occ_mu;
+  std::lock_guard<std::mutex> lock(occ_mu);
   auto it = occ.find(kernel);
   if (it == occ.end())
   {
