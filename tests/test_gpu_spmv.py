@@ -489,3 +489,41 @@ def test_complex_update_and_reverse(tdt):
     for j, t in enumerate(idx):  # sequential order of L2GMap.cpp:168-169
         acc[t] = acc[t] + inc[j]
     assert np.array_equal(vd.cpu().numpy()[:n], acc[:n])
+
+
+def test_create_like_copies_from_two_threads():
+    """Matrices made by create_like share their pattern; two host threads
+    that select different kernel variants (each building its own row blocks
+    on the shared pattern on first use) give the sequential results."""
+    import threading
+    A = P.stencil(27, 24, 24, 24)
+    M64 = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    x = P.uniform(31, A.ncols, -1, 1)
+    want64 = O.spmv(_ocsr(A), x)
+    xf = x.astype(np.float32)
+    want32 = O.spmv_f32(_ocsr(A), A.val.astype(np.float32), xf)
+    errors = []
+
+    def work(M, xv, want, variants):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for v in variants:
+                    M.set_kernel(v)
+                    y = M.mult(xv, stream=s)
+                    s.synchronize()
+                    if not np.array_equal(y.cpu().numpy(), want):
+                        errors.append(("variant", v))
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(repr(e))
+
+    M32 = M64.like("f32")
+    xd, xfd = _dev(x), _dev(xf, torch.float32)
+    torch.cuda.synchronize()  # inputs on the device before the side streams read them
+    t1 = threading.Thread(target=work, args=(M64, xd, want64, [0, 5, 8, 11, 14, 13, 9]))
+    t2 = threading.Thread(target=work, args=(M32, xfd, want32, [12, 3, 7, 10, 1, 14, 6]))
+    t1.start()
+    t2.start()
+    t1.join()
+    t2.join()
+    assert not errors, errors
