@@ -124,6 +124,23 @@ def test_mixed_precision_refinement(dtype):
     assert np.linalg.norm(xg - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("variant", range(S.SLICED_VARIANT))
+def test_cg_and_refinement_every_csr_variant(variant):
+    """Every CSR kernel kind in the CG modes (SpMV + p.y, residual b - Ax):
+    FP64 CG to the oracle's contract, and refinement with a BF16 inner matrix
+    using the same variant."""
+    A = P.stencil(7, 20, 20, 20)
+    b = P.uniform(14, A.nrows, -1, 1)
+    check_cg(A, b, 10000, 1e-10, variant=variant)
+    M64 = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    M64.set_kernel(variant)
+    Ml = M64.like("bf16")
+    Ml.set_kernel(variant)
+    x, st = S.ir_cg(M64, Ml, _dev(b), rtol=1e-10, inner_rtol=1e-4)
+    r = b - O.spmv(_ocsr(A), x.cpu().numpy())
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b)
+
+
 def test_cg_f32_inner_solver():
     A = P.stencil(5, 40, 40)
     b = P.uniform(2, A.nrows, -1, 1).astype(np.float32)
