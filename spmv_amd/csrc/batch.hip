@@ -681,11 +681,7 @@ int spmvx_batch_solve(spmvx_batch B, const float* b, float* x, int32_t max_its, 
         || B->cap_rtol2 != rtol2)
     {
       drop_graph(B);
-      SPMVX_HIP(hipStreamBeginCapture(B->own, hipStreamCaptureModeThreadLocal));
-      launch_batch(B, b, x, max_its, rtol2, B->own);
-      hipGraph_t g = nullptr;
-      SPMVX_HIP(hipStreamEndCapture(B->own, &g));
-      B->graph = g;
+      B->graph = capture_graph(B->own, [&] { launch_batch(B, b, x, max_its, rtol2, B->own); });
       SPMVX_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
       B->cap_b = b;
       B->cap_x = x;

@@ -101,7 +101,12 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
   }
   hipStream_t q = w.stream;
   if (col_map)
+  {
     l2gmap_prepare_device(col_map);
+    // checked here, not first inside the captured iterations
+    SPMVX_REQUIRE(col_map->halo_dtype < 0 || col_map->halo_dtype == vec_dtype,
+                  "halo dtype differs from the vector dtype");
+  }
   const bool sliced = A->variant == kSlicedVariant;
   const SlicedView sv = sliced ? sliced_view(A) : SlicedView{};
   const void* vals = sliced ? A->svalues.ptr : A->values.ptr;
@@ -136,14 +141,11 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
       SPMVX_HIP(hipGraphDestroy(w.graph));
     w.exec = nullptr;
     w.graph = nullptr;
-    SPMVX_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < kGraphChunk; ++j)
-      iteration(q);
-    hipGraph_t g = nullptr;
-    const hipError_t cap = hipStreamEndCapture(q, &g);
+    w.graph = capture_graph(q, [&] {
+      for (int j = 0; j < kGraphChunk; ++j)
+        iteration(q);
+    });
     check_launch("cg iteration capture");
-    SPMVX_HIP(cap);
-    w.graph = g;
     SPMVX_HIP(hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0));
     w.g_x = x;
     w.g_hist = hist_dev;

@@ -150,6 +150,29 @@ inline const RowBlocks& variant_blocks(Pattern& P, int var)
 // is missing; throws with the reason when the pattern is not eligible.
 SlicedView sliced_view(spmvx_matrix_s* A);
 
+// Records the launches of f on stream q (capture mode thread-local) as a
+// graph.  If f throws, the capture is ended and its partial graph dropped
+// before the exception propagates, so the stream stays usable.
+template <typename F>
+hipGraph_t capture_graph(hipStream_t q, F&& f)
+{
+  SPMVX_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
+  try
+  {
+    f();
+  }
+  catch (...)
+  {
+    hipGraph_t partial = nullptr;
+    if (hipStreamEndCapture(q, &partial) == hipSuccess && partial)
+      (void)hipGraphDestroy(partial);
+    throw;
+  }
+  hipGraph_t g = nullptr;
+  SPMVX_HIP(hipStreamEndCapture(q, &g));
+  return g;
+}
+
 // Grid of the vector kernels for n elements.
 inline int vec_grid(int64_t n)
 {

@@ -72,6 +72,23 @@ def test_cg_with_ghost_plane_zero_halo():
     check_cg(A, b, 2000, 1e-10, col_map=m)
 
 
+def test_cg_rejects_halo_of_another_dtype_and_recovers():
+    """A column map holding an FP32 halo cannot feed FP64 CG: the error is
+    raised before any iteration is recorded, and the same matrix then solves
+    normally once the halo is cleared."""
+    A = P.stencil(27, 10, 10, 10, ghost_planes=1)
+    m = S.L2GMap(**P.slab_decomposition(10, 10, 10))
+    m.set_halo(P.uniform(5, m.num_ghosts(), -1, 1), dtype=S.F32)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, col_map=m)
+    b = _dev(P.uniform(6, A.nrows, -1, 1))
+    with pytest.raises(S.SpmvError, match="halo dtype"):
+        S.cg(M, b, 100, 1e-10)
+    m.set_halo(None)
+    x, k = S.cg(M, b, 2000, 1e-10)
+    _, ko, _ = O.cg(_ocsr(A), b.cpu().numpy(), 2000, 1e-10)
+    assert abs(k - ko) <= 1
+
+
 def test_cg_eigenvector_one_iteration():
     n = 30
     A = P.stencil(5, n, n)
