@@ -213,13 +213,15 @@ int spmvx_matrix_set_values(spmvx_matrix A, const double* values_dev, spmvx_stre
  * y[nrows] = A x[ncols] -- Matrix<T>::operator* (Matrix.h:37-39,
  * Matrix.cpp:158-166,199-204).  For SPMVX_F64 every row is summed in stored
  * column order with separately rounded products and sums, starting from 0:
- * the result is bit-identical to the reference's Eigen product.
+ * the result is bit-identical to the reference's Eigen product.  x and y
+ * must not overlap (SPMVX_ERR_ARG).
  */
 int spmvx_spmv(spmvx_matrix A, const void* x_dev, void* y_dev, spmvx_stream stream);
 
 /* y[ncols] = A^T x[nrows] -- Matrix<T>::transpmult (Matrix.h:41-43,
  * Matrix.cpp:206-211).  Uses an analysis-time transposed copy; every output
- * entry is summed in ascending row order (bit-identical for SPMVX_F64). */
+ * entry is summed in ascending row order (bit-identical for SPMVX_F64).
+ * x and y must not overlap. */
 int spmvx_spmv_t(spmvx_matrix A, const void* x_dev, void* y_dev, spmvx_stream stream);
 
 /* ===================================================================== */

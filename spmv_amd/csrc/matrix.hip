@@ -724,12 +724,31 @@ int spmvx_matrix_sliced_info(spmvx_matrix A, int32_t* npatterns, int32_t* wmax, 
   });
 }
 
+namespace
+{
+// Bytes of one vector element of a matrix of `dtype` (BF16 matrices take
+// FP32 vectors).
+size_t vec_bytes(int32_t dtype)
+{
+  return dtype == SPMVX_BF16 ? 4 : dtype_size(dtype);
+}
+
+// The product reads x while it writes y: the two ranges must not overlap.
+void require_disjoint(const void* x, int64_t nx, const void* y, int64_t ny, size_t es)
+{
+  const auto xa = reinterpret_cast<uintptr_t>(x), ya = reinterpret_cast<uintptr_t>(y);
+  const uintptr_t xe = xa + static_cast<uintptr_t>(nx) * es, ye = ya + static_cast<uintptr_t>(ny) * es;
+  SPMVX_REQUIRE(nx == 0 || ny == 0 || xe <= ya || ye <= xa, "x and y must not overlap");
+}
+} // namespace
+
 int spmvx_spmv(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
 {
   return guarded([&] {
     SPMVX_REQUIRE(A != nullptr, "NULL matrix");
     SPMVX_REQUIRE(A->pat->nrows == 0 || y != nullptr, "y is NULL");
     SPMVX_REQUIRE(A->pat->nnz == 0 || x != nullptr, "x is NULL");
+    require_disjoint(x, A->pat->ncols, y, A->pat->nrows, vec_bytes(A->dtype));
     launch_matrix_spmv(A, x, y, as_stream(stream));
   });
 }
@@ -738,9 +757,11 @@ int spmvx_spmv_t(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
 {
   return guarded([&] {
     SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    SPMVX_REQUIRE(A->pat->ncols == 0 || y != nullptr, "y is NULL");
+    SPMVX_REQUIRE(A->pat->nnz == 0 || x != nullptr, "x is NULL");
+    require_disjoint(x, A->pat->nrows, y, A->pat->ncols, vec_bytes(A->dtype));
     if (!A->tpat)
       build_transpose(A);
-    SPMVX_REQUIRE(A->pat->ncols == 0 || y != nullptr, "y is NULL");
     launch_spmv(A->dtype, csr_variant(A), *A->tpat, A->tvalues.ptr, x, y, as_stream(stream),
                 A->long_rows);
   });

@@ -527,3 +527,19 @@ def test_create_like_copies_from_two_threads():
     t1.join()
     t2.join()
     assert not errors, errors
+
+
+def test_products_reject_overlapping_vectors():
+    """The product reads x while it writes y; overlapping ranges are an
+    argument error, adjacent ones are fine."""
+    A = P.stencil(5, 8, 8)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    x = _dev(P.uniform(41, A.ncols, -1, 1))
+    with pytest.raises(S.SpmvError, match="overlap"):
+        M.mult(x, out=x)
+    with pytest.raises(S.SpmvError, match="overlap"):
+        M.transpmult(x, out=x)
+    buf = torch.zeros(A.ncols + A.nrows, dtype=torch.float64, device=DEV)
+    buf[:A.ncols] = x
+    y = M.mult(buf[:A.ncols], out=buf[A.ncols:])
+    assert np.array_equal(y.cpu().numpy(), O.spmv(_ocsr(A), x.cpu().numpy()))
