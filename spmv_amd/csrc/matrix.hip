@@ -426,8 +426,10 @@ void validate_csr(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* rowp
                   const int32_t* colidx)
 {
   SPMVX_REQUIRE(nrows >= 0 && ncols >= 0 && nnz >= 0, "negative matrix size");
+  // entry offsets in the kernels run up to one staging step (at most 64 Ki
+  // entries) past nnz in 32-bit arithmetic
   SPMVX_REQUIRE(nrows < (int64_t(1) << 31) && ncols < (int64_t(1) << 31)
-                    && nnz < (int64_t(1) << 31),
+                    && nnz <= (int64_t(1) << 31) - (int64_t(1) << 16),
                 "matrix too large for 32-bit indices");
   SPMVX_REQUIRE(rowptr != nullptr, "rowptr is NULL");
   SPMVX_REQUIRE(nnz == 0 || colidx != nullptr, "colidx is NULL");
