@@ -112,6 +112,24 @@ def test_spmv_bitexact_vs_scipy(A):
     assert np.all(np.abs(y - yd) <= 4e-16 * A.ncols * scale + 1e-300)
 
 
+@pytest.mark.parametrize("A", list(_cases()))
+def test_spmv_f32_bitexact_vs_scipy(A):
+    """The FP32 product (reference for the FP32/BF16 kernels) is scipy's
+    float32 csr_matvec: products and sums rounded to float32 in stored
+    order."""
+    import scipy.sparse as sp
+    x = O.uniform(12, A.ncols, -1, 1).astype(np.float32)
+    v = A.val.astype(np.float32)
+    S32 = sp.csr_matrix((v, A.colidx, A.rowptr), shape=(A.nrows, A.ncols))
+    y = O.spmv_f32(A, v, x)
+    assert y.dtype == np.float32
+    assert np.array_equal(y, S32 @ x)
+    # the same rounded inputs summed in float64 differ somewhere: the oracle
+    # really rounds its products and sums to float32
+    y64 = S32.astype(np.float64) @ x.astype(np.float64)
+    assert not np.array_equal(y.astype(np.float64), y64)
+
+
 @pytest.mark.parametrize("kind,shape", [(5, (9, 7)), (7, (5, 6, 4)), (27, (4, 5, 6))])
 def test_spmv_closed_form_eigenvectors(kind, shape):
     nx, ny = shape[0], shape[1]
