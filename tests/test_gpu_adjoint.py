@@ -91,3 +91,31 @@ def test_gradcheck_tiny():
     th = torch.from_numpy(theta).to(DEV).requires_grad_(True)
     bd = torch.from_numpy(b).to(DEV).requires_grad_(True)
     assert torch.autograd.gradcheck(f, (th, bd), eps=1e-6, atol=1e-6, rtol=1e-5)
+
+
+def test_config_diffusion_1024_full_size():
+    """The 1024^2 anisotropic diffusion problem at full size (5,238,784
+    entries): the adjoint solution satisfies A lambda = x - x_target to FP64
+    CG's attainable accuracy, and
+    central differences (eps 1e-6) on 4 seeded entries match the gradient to
+    1e-6 relative."""
+    n = 1024
+    theta, b, xt = problem(n, 1)
+    op = AD.DiffusionOperator(n)
+    assert op.A.nnz == 5_238_784
+    th = torch.from_numpy(theta).to(DEV).requires_grad_(True)
+    bd = torch.from_numpy(b).to(DEV).requires_grad_(True)
+    xtd = torch.from_numpy(xt).to(DEV)
+    x = op.solve(th, bd, rtol=1e-13)
+    (0.5 * ((x - xtd) ** 2).sum()).backward()
+    lam = bd.grad.detach().cpu().numpy()          # grad_b = lambda
+    A = O.gen_diffusion(n, theta)
+    rhs_adj = (x.detach() - xtd).cpu().numpy()
+    r = O.spmv(O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val), lam, threads=8) - rhs_adj
+    # the recursive residual stops at 1e-13; the true residual of FP64 CG on
+    # this operator (condition number ~1e6-1e7) floors near kappa * eps ~ 1e-9
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(rhs_adj)
+    entries = (P.rand_u64(77, 4) % np.uint64(op.num_edges)).astype(np.int64)
+    fd = AD.central_difference(op, th.detach(), bd.detach(), xtd, entries, eps=1e-6, rtol=1e-13)
+    g = th.grad.cpu().numpy()[entries]
+    assert np.max(np.abs(fd - g) / np.maximum(np.abs(g), 1e-300)) <= 1e-6

@@ -110,3 +110,26 @@ def test_autotuner_caches(tmp_path, monkeypatch):
         (best["block"], best["lanes_per_row"], best["rows_per_member"])
     best2, table2 = T.tune_batch(B, bd, max_its=500, rtol=1e-6, reps=1)
     assert table2 is None and best2 == best
+
+
+def test_config_batch_full_size():
+    """The full batch: 256 SPD systems, rows log-uniform in [1e3, 2e5] (9.9M
+    rows), 4 right-hand sides each, one persistent launch to 1e-6; every
+    right-hand side converges, and the true residuals of every 32nd system
+    meet the FP32 accuracy bound."""
+    nsys, nrhs = 256, 4
+    ns = np.exp(np.log(1e3) + (np.log(2e5) - np.log(1e3)) * P.uniform(17, nsys)).astype(np.int64)
+    systems = [P.random_spd(int(n), 1000 + i) for i, n in enumerate(ns)]
+    B = S.Batch(systems, nrhs=nrhs)
+    B.set_config(1024, 4, 4096)
+    bh = rhs(B.nrows, nrhs, 23)
+    x, its = B.solve(torch.from_numpy(bh).to(DEV), max_its=20000, rtol=1e-6)
+    assert np.all(its > 0) and np.all(its < 20000)
+    xh = x.cpu().numpy()
+    for s in range(0, nsys, 32):
+        A = systems[s]
+        lo, hi = B.sys_row[s], B.sys_row[s + 1]
+        oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
+        for j in range(nrhs):
+            r = bh[lo:hi, j].astype(np.float64) - O.spmv(oc, xh[lo:hi, j].astype(np.float64))
+            assert np.linalg.norm(r) <= 5e-6 * np.linalg.norm(bh[lo:hi, j]), (s, j)
