@@ -543,3 +543,44 @@ def test_products_reject_overlapping_vectors():
     buf[:A.ncols] = x
     y = M.mult(buf[:A.ncols], out=buf[A.ncols:])
     assert np.array_equal(y.cpu().numpy(), O.spmv(_ocsr(A), x.cpu().numpy()))
+
+
+def boundary_matrix(seed):
+    """Random CSR whose row lengths sit on and around every kernel's block
+    limits (2040/2048, 4088/4096, 8184/8192 entries), mixed with empty rows
+    and runs of short rows, so blocks end exactly at, just before and just
+    after their limits."""
+    rng = np.random.default_rng(seed)
+    ncols = int(rng.integers(9000, 20000))
+    specials = [0, 1, 7, 2039, 2040, 2041, 2047, 2048, 2049, 4087, 4088, 4089, 4095, 4096, 4097,
+                8183, 8184, 8185, 8191, 8192, 8193]
+    # short rows (10% empty), then every special length twice at random places
+    lens = list(np.where(rng.random(3000) < 0.1, 0, rng.integers(1, 60, 3000)))
+    for ln in specials + specials:
+        lens.insert(int(rng.integers(0, len(lens) + 1)), ln)
+    lens = np.minimum(np.array(lens, np.int64), ncols)
+    rowptr = np.zeros(len(lens) + 1, np.int64)
+    rowptr[1:] = np.cumsum(lens)
+    colidx = np.concatenate([np.sort(rng.choice(ncols, l, replace=False)) if l else
+                             np.zeros(0, np.int64) for l in lens]).astype(np.int32)
+    val = P.uniform(seed + 50, len(colidx), -1, 1)
+    return P.CSR(len(lens), ncols, rowptr.astype(np.int32), colidx, val)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5, 6])
+def test_block_boundary_rows_every_variant(seed):
+    A = boundary_matrix(seed)
+    x = P.uniform(seed + 60, A.ncols, -1, 1)
+    want = O.spmv(_ocsr(A), x)
+    want_t = O.spmv_t(_ocsr(A), P.uniform(seed + 61, A.nrows, -1, 1))
+    xf = x.astype(np.float32)
+    want32 = O.spmv_f32(_ocsr(A), A.val.astype(np.float32), xf)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    Mf = M.like("f32")
+    for v in range(S.SLICED_VARIANT):
+        M.set_kernel(v)
+        assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), want), (seed, v)
+        Mf.set_kernel(v)
+        assert np.array_equal(Mf.mult(_dev(xf, torch.float32)).cpu().numpy(), want32), (seed, v)
+    assert np.array_equal(M.transpmult(_dev(P.uniform(seed + 61, A.nrows, -1, 1))).cpu().numpy(),
+                          want_t)
