@@ -367,9 +367,37 @@ void launch_plain(int var, Pattern& P, const void* vals, const void* x, void* y,
   check_launch("csr SpMV");
 }
 
+// Bytes of one vector element of a matrix of `dtype` (BF16 matrices take
+// FP32 vectors).
+size_t vec_bytes(int32_t dtype)
+{
+  return dtype == SPMVX_BF16 ? 4 : dtype_size(dtype);
+}
+
+// w[0:n) = 0 (a kernel, so a caller may capture it in a graph)
+__global__ void zero_words(uint32_t* __restrict__ w, int64_t n)
+{
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+    w[i] = 0u;
+}
+
 void launch_spmv(int32_t dtype, int var, Pattern& P, const void* vals, const void* x, void* y,
                  hipStream_t s, int32_t long_par)
 {
+  if (P.nnz == 0)
+  {
+    // no stored entries: y = 0.  The product kernels gather x at padding
+    // columns without a branch, which needs x to hold at least one entry;
+    // a matrix without entries may have no columns at all.
+    const int64_t words = P.nrows * static_cast<int64_t>(vec_bytes(dtype)) / 4;
+    if (words > 0)
+    {
+      zero_words<<<vec_grid(words), kVecBS, 0, s>>>(static_cast<uint32_t*>(y), words);
+      check_launch("zero_words");
+    }
+    return;
+  }
   if (dtype == SPMVX_F64)
     launch_plain<double, double, double>(var, P, vals, x, y, s, long_par);
   else if (dtype == SPMVX_F32)
@@ -728,13 +756,6 @@ int spmvx_matrix_sliced_info(spmvx_matrix A, int32_t* npatterns, int32_t* wmax, 
 
 namespace
 {
-// Bytes of one vector element of a matrix of `dtype` (BF16 matrices take
-// FP32 vectors).
-size_t vec_bytes(int32_t dtype)
-{
-  return dtype == SPMVX_BF16 ? 4 : dtype_size(dtype);
-}
-
 // The product reads x while it writes y: the two ranges must not overlap.
 void require_disjoint(const void* x, int64_t nx, const void* y, int64_t ny, size_t es)
 {

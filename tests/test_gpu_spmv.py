@@ -584,3 +584,21 @@ def test_block_boundary_rows_every_variant(seed):
         assert np.array_equal(Mf.mult(_dev(xf, torch.float32)).cpu().numpy(), want32), (seed, v)
     assert np.array_equal(M.transpmult(_dev(P.uniform(seed + 61, A.nrows, -1, 1))).cpu().numpy(),
                           want_t)
+
+
+def test_products_of_matrices_without_entries():
+    """No stored entries: y = 0, including a matrix with rows but no columns
+    (x empty) and the transposed product of a matrix without rows; no product
+    kernel runs (its branch-free gathers would need x to hold an entry)."""
+    M = S.Matrix(np.zeros(6, np.int32), np.zeros(0, np.int32), np.zeros(0), ncols=0)
+    y = M.mult(torch.zeros(0, dtype=torch.float64, device=DEV))
+    assert y.shape == (5,) and torch.all(y == 0)
+    Mf = M.like("f32")
+    assert torch.all(Mf.mult(torch.zeros(0, dtype=torch.float32, device=DEV)) == 0)
+    Mt = S.Matrix(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0), ncols=4)
+    yt = Mt.transpmult(torch.zeros(0, dtype=torch.float64, device=DEV))
+    assert yt.shape == (4,) and torch.all(yt == 0)
+    Ms = S.Matrix(np.zeros(7, np.int32), np.zeros(0, np.int32), np.zeros(0), ncols=6)
+    out = torch.full((6,), 3.0, dtype=torch.float64, device=DEV)
+    Ms.mult(torch.ones(6, dtype=torch.float64, device=DEV), out=out)
+    assert torch.all(out == 0)
