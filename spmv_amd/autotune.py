@@ -96,7 +96,9 @@ def tune_batch(batch, b, max_its=10000, rtol=1e-6, reps=2, blocks=BATCH_BLOCKS,
 
 
 def spmv_signature(A):
-    return f"spmv:{_device_name()}:{A.dtype}:{A.nrows}:{A.ncols}:{A.nnz}"
+    from . import lib
+    return (f"spmv:{_device_name()}:{lib().spmvx_num_kernels()}:{A.dtype}:"
+            f"{A.nrows}:{A.ncols}:{A.nnz}")
 
 
 def tune_spmv(A, x, reps=10, use_cache=True):
@@ -104,13 +106,20 @@ def tune_spmv(A, x, reps=10, use_cache=True):
     results), applies the fastest and returns (best, table)."""
     import torch
     from . import lib
+    from . import SpmvError
     key = spmv_signature(A)
     if use_cache:
         hit = _load().get(key)
         if hit:
-            A.set_kernel(hit["variant"])
-            return hit, None
-    from . import SpmvError
+            # the signature is the shape only: a choice stored for another
+            # matrix of this shape (a layout this one does not fit) or by a
+            # build whose variant table differed is tuned again
+            try:
+                A.set_kernel(hit["variant"])
+                if hit.get("kernel") == A.kernel()[1]:
+                    return hit, None
+            except SpmvError:
+                pass
     y = torch.empty(A.nrows, dtype=A.vector_dtype(), device=x.device)
     table = []
     for v in range(lib().spmvx_num_kernels()):
@@ -121,7 +130,8 @@ def tune_spmv(A, x, reps=10, use_cache=True):
         A.mult(x, out=y)
         ms = _time_ms(lambda: A.mult(x, out=y), reps)
         table.append({"variant": v, "ms": ms})
-    best = min(table, key=lambda r: r["ms"])
+    best = dict(min(table, key=lambda r: r["ms"]))
     A.set_kernel(best["variant"])
+    best["kernel"] = A.kernel()[1]
     _store(key, best)
     return best, table

@@ -602,3 +602,36 @@ def test_products_of_matrices_without_entries():
     out = torch.full((6,), 3.0, dtype=torch.float64, device=DEV)
     Ms.mult(torch.ones(6, dtype=torch.float64, device=DEV), out=out)
     assert torch.all(out == 0)
+
+
+def test_spmv_autotuner_cache(tmp_path, monkeypatch):
+    """tune_spmv times every variant the matrix fits, applies the fastest
+    and caches it; a second call uses the cache; a cached choice that does
+    not apply to the matrix at hand (the sliced layout for a matrix it does
+    not fit, or a variant whose kernel differs from the stored one) is tuned
+    again instead of failing.  Results are bit-exact throughout."""
+    from spmv_amd import autotune as T
+    monkeypatch.setenv("SPMV_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    A = P.stencil(27, 20, 20, 20)
+    x = P.uniform(51, A.ncols, -1, 1)
+    want = O.spmv(_ocsr(A), x)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    best, table = T.tune_spmv(M, _dev(x), reps=2)
+    assert table is not None and len(table) == S.lib().spmvx_num_kernels()  # the stencil fits all
+    assert M.kernel() == (best["variant"], best["kernel"])
+    assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), want)
+    best2, table2 = T.tune_spmv(M, _dev(x), reps=2)
+    assert table2 is None and best2 == best
+    # planted entries for a matrix of the same signature
+    R = P.random_spd(3000, 8)
+    MR = S.Matrix(R.rowptr, R.colidx, R.val, ncols=R.ncols)
+    key = T.spmv_signature(MR)
+    xr = P.uniform(52, R.ncols, -1, 1)
+    for planted in ({"variant": S.SLICED_VARIANT, "kernel": "sliced", "ms": 0.0},
+                    {"variant": 0, "kernel": "not this kernel", "ms": 0.0}):
+        T._store(key, planted)
+        bestr, tabler = T.tune_spmv(MR, _dev(xr), reps=1)
+        assert tabler is not None
+        assert all(r["variant"] != S.SLICED_VARIANT for r in tabler)  # refused, skipped
+        assert MR.kernel() == (bestr["variant"], bestr["kernel"])
+        assert np.array_equal(MR.mult(_dev(xr)).cpu().numpy(), O.spmv(_ocsr(R), xr))
