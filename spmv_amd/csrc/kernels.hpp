@@ -115,16 +115,19 @@ template <typename Acc>
 __device__ __forceinline__ Acc finalize_sum(const Acc* __restrict__ partials, int64_t n,
                                             Acc* red)
 {
-  // 16 independent loads per thread and step (latency-bound on tens of
-  // thousands of partials), four accumulators, fixed order
+  // 16 independent loads per thread and step, the last step predicated
+  // (latency-bound: tens of thousands of partials take one to four round
+  // trips), four accumulators, fixed order
   Acc s0 = Acc(0), s1 = Acc(0), s2 = Acc(0), s3 = Acc(0);
-  int64_t i = threadIdx.x;
-  for (; i + 15 * kFinBS < n; i += 16 * kFinBS)
+  for (int64_t i = threadIdx.x; i < n; i += 16 * kFinBS)
   {
     Acc v[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u)
-      v[u] = partials[i + u * kFinBS];
+    {
+      const int64_t j = i + u * kFinBS;
+      v[u] = j < n ? partials[j] : Acc(0);
+    }
 #pragma unroll
     for (int u = 0; u < 16; u += 4)
     {
@@ -134,8 +137,6 @@ __device__ __forceinline__ Acc finalize_sum(const Acc* __restrict__ partials, in
       s3 += v[u + 3];
     }
   }
-  for (; i < n; i += kFinBS)
-    s0 += partials[i];
   return block_sum((s0 + s1) + (s2 + s3), red);
 }
 
