@@ -364,8 +364,14 @@ int spmvx_sddmm(spmvx_matrix A, double alpha, const double* u_dev, const double*
 /* Utilities                                                              */
 /* ===================================================================== */
 /* dst[i] = src[i], n doubles: the known-size copy kernel used to calibrate
- * HBM byte counters. */
+ * HBM byte counters (and the read+write ceiling).  No counterpart in the
+ * reference (a measurement utility). */
 int spmvx_copy_f64(double* dst_dev, const double* src_dev, int64_t n, spmvx_stream stream);
+/* *sum_dev = the sum of n doubles (summation order unspecified), read once
+ * with non-temporal 16-byte loads: the streaming-read ceiling of this GPU,
+ * against which the SpMV kernels (~97% reads) are measured.  src_dev 16-byte
+ * aligned.  No counterpart in the reference (a measurement utility). */
+int spmvx_read_f64(const double* src_dev, int64_t n, double* sum_dev, spmvx_stream stream);
 
 #ifdef __cplusplus
 }

@@ -23,7 +23,7 @@ import os
 
 import numpy as np
 
-__all__ = ["L2GMap", "Matrix", "Batch", "cg", "cg_f32", "ir_cg", "sddmm", "copy_f64", "SpmvError",
+__all__ = ["L2GMap", "Matrix", "Batch", "cg", "cg_f32", "ir_cg", "sddmm", "copy_f64", "read_f64", "SpmvError",
            "read_petsc_csr", "read_petsc_matrix", "read_petsc_vector",
            "F64", "F32", "BF16", "C64", "C32", "SLICED_VARIANT", "lib", "library_path"]
 
@@ -108,6 +108,7 @@ def lib():
         "spmvx_ir_cg": ([P, P, P, P, i32, f64, i32, f64, i32, P, P], ctypes.c_int),
         "spmvx_sddmm": ([P, f64, P, P, P, P], ctypes.c_int),
         "spmvx_copy_f64": ([P, P, i64, P], ctypes.c_int),
+        "spmvx_read_f64": ([P, i64, P, P], ctypes.c_int),
         "spmvx_petsc_csr_read": ([ctypes.c_char_p, i32, i32, P], ctypes.c_int),
         "spmvx_petsc_csr_destroy": ([P], ctypes.c_int),
         "spmvx_petsc_csr_info": ([P, P, P, P, P, P, P], ctypes.c_int),
@@ -488,6 +489,17 @@ def copy_f64(dst, src, stream=None):
     if dst.numel() < src.numel():
         raise SpmvError("dst is shorter than src")
     _check(lib().spmvx_copy_f64(_dptr(dst), _dptr(src), src.numel(), _stream(stream)))
+
+
+def read_f64(src, out, stream=None):
+    """out[0] = sum of the float64 vector src, read once with streaming
+    (non-temporal) loads: the streaming-read ceiling of the GPU."""
+    import torch
+    if src.dtype != torch.float64 or out.dtype != torch.float64:
+        raise SpmvError("read_f64 takes float64 tensors")
+    if out.numel() < 1:
+        raise SpmvError("out must hold one value")
+    _check(lib().spmvx_read_f64(_dptr(src), src.numel(), _dptr(out), _stream(stream)))
 
 
 # =========================================================================

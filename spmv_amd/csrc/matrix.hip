@@ -298,6 +298,41 @@ __global__ void copy_f64_kernel(double* __restrict__ dst, const double* __restri
     dst[n - 1] = src[n - 1];
 }
 
+// Streaming read of n doubles: non-temporal 16-byte loads, four per thread in
+// flight per step, the values summed (one atomic add per workgroup into *sum)
+// so that no load is dead code.  The read ceiling of the SpMV kernels, which
+// read ~97% of their bytes.
+constexpr int kReadUnroll = 4;
+__global__ __launch_bounds__(256) void read_f64_kernel(const double* __restrict__ src, int64_t n,
+                                                       double* __restrict__ sum)
+{
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2* s2 = reinterpret_cast<const d2*>(src);
+  const int64_t n2 = n / 2;
+  const int64_t tile = static_cast<int64_t>(gridDim.x) * 256 * kReadUnroll;
+  d2 acc = {0.0, 0.0};
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * 256 * kReadUnroll; base < n2; base += tile)
+  {
+    d2 v[kReadUnroll];
+#pragma unroll
+    for (int u = 0; u < kReadUnroll; ++u)
+    {
+      const int64_t i = base + u * 256 + threadIdx.x;
+      v[u] = i < n2 ? __builtin_nontemporal_load(s2 + i) : d2{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < kReadUnroll; ++u)
+      acc += v[u];
+  }
+  double t = acc.x + acc.y;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1))
+    t += src[n - 1];
+  __shared__ double red[16];
+  t = block_sum(t, red);
+  if (threadIdx.x == 0)
+    atomicAdd(sum, t);
+}
+
 // dst[slot(row, k)] = src[rowptr[row] + k]: CSR values into the sliced
 // row-pattern layout (padding slots keep the zeros they were allocated with).
 template <typename T>
@@ -823,6 +858,30 @@ int spmvx_copy_f64(double* dst, const double* src, int64_t n, spmvx_stream strea
     if (grid < 1) grid = 1;
     copy_f64_kernel<<<dim3(static_cast<unsigned>(grid)), dim3(256), 0, as_stream(stream)>>>(dst, src, n);
     check_launch("copy_f64_kernel");
+  });
+}
+
+int spmvx_read_f64(const double* src, int64_t n, double* sum, spmvx_stream stream)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(n >= 0, "negative size");
+    SPMVX_REQUIRE(sum, "NULL pointer");
+    SPMVX_REQUIRE(n == 0 || src, "NULL pointer");
+    SPMVX_REQUIRE((reinterpret_cast<uintptr_t>(src) & 15) == 0
+                      && (reinterpret_cast<uintptr_t>(sum) & 7) == 0,
+                  "read needs a 16-byte aligned source and an 8-byte aligned sum");
+    hipStream_t s = as_stream(stream);
+    zero_words<<<1, kVecBS, 0, s>>>(reinterpret_cast<uint32_t*>(sum), 2);
+    check_launch("zero_words");
+    if (n == 0)
+      return;
+    int64_t grid = (n / 2 + 256 * kReadUnroll - 1) / (256 * kReadUnroll);
+    if (grid > 1024)
+      grid = 1024;
+    if (grid < 1)
+      grid = 1;
+    read_f64_kernel<<<dim3(static_cast<unsigned>(grid)), dim3(256), 0, s>>>(src, n, sum);
+    check_launch("read_f64_kernel");
   });
 }
 
