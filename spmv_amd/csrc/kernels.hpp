@@ -1123,17 +1123,30 @@ int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X*
   if (nblk <= 0)
     return 0;
   const dim3 grid(static_cast<unsigned>(nblk));
-#define SPMVX_SLICED_LAUNCH(R)                                                                \
-  sliced_spmv<V, X, Acc, MODE, UNR, R><<<grid, dim3(kSliceBS), lds, s>>>(                    \
-      S.pid, S.sdesc, S.nslices, S.nrows, S.ncols, S.dict, S.dict_n, S.wmax, val, x, y, b,  \
-      partials, st, it)
+  auto go = [&](auto unr, auto rpl) {
+    sliced_spmv<V, X, Acc, MODE, decltype(unr)::value, decltype(rpl)::value>
+        <<<grid, dim3(kSliceBS), lds, s>>>(S.pid, S.sdesc, S.nslices, S.nrows, S.ncols, S.dict,
+                                           S.dict_n, S.wmax, val, x, y, b, partials, st, it);
+  };
+  using std::integral_constant;
   if (S.rpl == 4)
-    SPMVX_SLICED_LAUNCH(4);
+    go(integral_constant<int, UNR>{}, integral_constant<int, 4>{});
   else if (S.rpl == 2)
-    SPMVX_SLICED_LAUNCH(2);
+    go(integral_constant<int, UNR>{}, integral_constant<int, 2>{});
+  else if constexpr (sizeof(V) == 8)
+  {
+    // one FP64 slice per wavefront: the entries loaded per step matched to
+    // the pattern width, so that a row takes one or two steps (measured on
+    // the 27- and 7-point stencils: 3-4% over four per step)
+    if (S.wmax > 16)
+      go(integral_constant<int, 14>{}, integral_constant<int, 1>{});
+    else if (S.wmax > 4)
+      go(integral_constant<int, 8>{}, integral_constant<int, 1>{});
+    else
+      go(integral_constant<int, UNR>{}, integral_constant<int, 1>{});
+  }
   else
-    SPMVX_SLICED_LAUNCH(1);
-#undef SPMVX_SLICED_LAUNCH
+    go(integral_constant<int, UNR>{}, integral_constant<int, 1>{});
   return nblk;
 }
 
