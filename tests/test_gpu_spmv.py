@@ -364,6 +364,37 @@ def test_sliced_layout_bitexact(name, A):
         assert np.array_equal(Ml.mult(xf).cpu().numpy(), ref), dt
 
 
+def band_matrix(n, lo, hi, seed):
+    """Rows i hold columns i-lo .. i+hi (clipped at the edges): one interior
+    pattern of width lo+hi+1 and lo+hi truncated ones."""
+    rows = [np.arange(max(0, i - lo), min(n, i + hi + 1), dtype=np.int32) for i in range(n)]
+    rowptr = np.zeros(n + 1, np.int32)
+    rowptr[1:] = np.cumsum([len(r) for r in rows])
+    colidx = np.concatenate(rows)
+    return P.CSR(n, n, rowptr, colidx, P.uniform(seed, len(colidx), -1, 1))
+
+
+@pytest.mark.parametrize("lo,hi", [(2, 2), (7, 7), (8, 7), (8, 8), (14, 14), (20, 20)],
+                         ids=["w5", "w15", "w16", "w17", "w29", "w41"])
+def test_sliced_layout_pattern_widths(lo, hi):
+    """Pattern widths on both sides of the FP64 kernel's per-step entry
+    counts (4 up to width 4, 8 up to 16, 14 above: one, two or three steps
+    with idle slots): FP64 bit-identical to the oracle, FP32/BF16 to the CSR
+    kernels."""
+    A = band_matrix(3001, lo, hi, seed=lo * 100 + hi)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    M.set_kernel(S.SLICED_VARIANT)
+    assert M.sliced_info()["wmax"] == lo + hi + 1
+    x = P.uniform(23, A.ncols, -1, 1)
+    assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), O.spmv(_ocsr(A), x))
+    xf = _dev(x, torch.float32)
+    for dt in ("f32", "bf16"):
+        Ml = M.like(dt)
+        ref = Ml.mult(xf).cpu().numpy()
+        Ml.set_kernel(S.SLICED_VARIANT)
+        assert np.array_equal(Ml.mult(xf).cpu().numpy(), ref), dt
+
+
 def test_sliced_layout_info():
     # 7-point stencil on a 13 x 12 x 11 grid: 3 boundary classes per axis
     A = P.stencil(7, 13, 12, 11)
