@@ -1129,22 +1129,55 @@ int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X*
                                            S.dict_n, S.wmax, val, x, y, b, partials, st, it);
   };
   using std::integral_constant;
+  constexpr int VW = sliced_vw<V>();
+  const int wpad = (S.wmax + VW - 1) / VW * VW; // slice width in slots
+  // Slots loaded per step at each value type's default slices per wavefront,
+  // chosen from the pattern width (measured on the 27- and 7-point stencils
+  // and on banded matrices of widths 5-41): FP64 up to 14 entries so that a
+  // row takes one to three steps; FP32 and BF16 the largest step that leaves
+  // no idle slot in the padded width.
+  if constexpr (sizeof(V) == 8)
+  {
+    if (S.rpl == 1)
+    {
+      if (S.wmax > 16)
+        go(integral_constant<int, 14>{}, integral_constant<int, 1>{});
+      else if (S.wmax > 4)
+        go(integral_constant<int, 8>{}, integral_constant<int, 1>{});
+      else
+        go(integral_constant<int, UNR>{}, integral_constant<int, 1>{});
+      return nblk;
+    }
+  }
+  else if constexpr (sizeof(V) == 4)
+  {
+    if (S.rpl == 4)
+    {
+      if (wpad % 4 == 0)
+        go(integral_constant<int, 2>{}, integral_constant<int, 4>{});
+      else
+        go(integral_constant<int, 1>{}, integral_constant<int, 4>{});
+      return nblk;
+    }
+  }
+  else
+  {
+    if (S.rpl == 2)
+    {
+      if (wpad % 12 == 0)
+        go(integral_constant<int, 3>{}, integral_constant<int, 2>{});
+      else if (wpad % 8 == 0)
+        go(integral_constant<int, 2>{}, integral_constant<int, 2>{});
+      else
+        go(integral_constant<int, 1>{}, integral_constant<int, 2>{});
+      return nblk;
+    }
+  }
+  // slices per wavefront set by hand (SPMVX_SLICED_RPL): the fixed step
   if (S.rpl == 4)
     go(integral_constant<int, UNR>{}, integral_constant<int, 4>{});
   else if (S.rpl == 2)
     go(integral_constant<int, UNR>{}, integral_constant<int, 2>{});
-  else if constexpr (sizeof(V) == 8)
-  {
-    // one FP64 slice per wavefront: the entries loaded per step matched to
-    // the pattern width, so that a row takes one or two steps (measured on
-    // the 27- and 7-point stencils: 3-4% over four per step)
-    if (S.wmax > 16)
-      go(integral_constant<int, 14>{}, integral_constant<int, 1>{});
-    else if (S.wmax > 4)
-      go(integral_constant<int, 8>{}, integral_constant<int, 1>{});
-    else
-      go(integral_constant<int, UNR>{}, integral_constant<int, 1>{});
-  }
   else
     go(integral_constant<int, UNR>{}, integral_constant<int, 1>{});
   return nblk;

@@ -374,13 +374,13 @@ def band_matrix(n, lo, hi, seed):
     return P.CSR(n, n, rowptr, colidx, P.uniform(seed, len(colidx), -1, 1))
 
 
-@pytest.mark.parametrize("lo,hi", [(2, 2), (7, 7), (8, 7), (8, 8), (14, 14), (20, 20)],
-                         ids=["w5", "w15", "w16", "w17", "w29", "w41"])
+@pytest.mark.parametrize("lo,hi", [(2, 2), (4, 4), (7, 7), (8, 7), (8, 8), (14, 14), (20, 20)],
+                         ids=["w5", "w9", "w15", "w16", "w17", "w29", "w41"])
 def test_sliced_layout_pattern_widths(lo, hi):
-    """Pattern widths on both sides of the FP64 kernel's per-step entry
-    counts (4 up to width 4, 8 up to 16, 14 above: one, two or three steps
-    with idle slots): FP64 bit-identical to the oracle, FP32/BF16 to the CSR
-    kernels."""
+    """Pattern widths on both sides of the kernels' per-step slot counts
+    (FP64: 4 up to width 4, 8 up to 16, 14 above, one to three steps with
+    idle slots; FP32: 2 or 4 slots, BF16: 4, 8 or 12 slots by the padded
+    width): FP64 bit-identical to the oracle, FP32/BF16 to the CSR kernels."""
     A = band_matrix(3001, lo, hi, seed=lo * 100 + hi)
     M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
     M.set_kernel(S.SLICED_VARIANT)
