@@ -18,6 +18,8 @@
 // one XCD are adjacent and share x entries in that XCD's L2.
 #pragma once
 
+#include <initializer_list>
+
 #include "common.hpp"
 
 namespace spmvx
@@ -1112,6 +1114,26 @@ struct SlicedView
 };
 
 // Launches the sliced kernel; returns the number of partial sums written.
+// Slots per step for a slice of w slots: among the candidate step sizes,
+// the one that leaves the fewest idle slots in the last step, then the one
+// with the fewest steps.
+inline int sliced_step(int w, std::initializer_list<int> sizes)
+{
+  int best = 0, best_idle = 0, best_steps = 0;
+  for (int u : sizes)
+  {
+    const int steps = (w + u - 1) / u;
+    const int idle = steps * u - w;
+    if (best == 0 || idle < best_idle || (idle == best_idle && steps < best_steps))
+    {
+      best = u;
+      best_idle = idle;
+      best_steps = steps;
+    }
+  }
+  return best;
+}
+
 template <typename V, typename X, typename Acc, int MODE>
 int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X* x, X* y,
                       const X* b, Acc* partials, const CgState<Acc>* st, int32_t it)
@@ -1132,20 +1154,23 @@ int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X*
   constexpr int VW = sliced_vw<V>();
   const int wpad = (S.wmax + VW - 1) / VW * VW; // slice width in slots
   // Slots loaded per step at each value type's default slices per wavefront,
-  // chosen from the pattern width (measured on the 27- and 7-point stencils
-  // and on banded matrices of widths 5-41): FP64 up to 14 entries so that a
-  // row takes one to three steps; FP32 and BF16 the largest step that leaves
-  // no idle slot in the padded width.
+  // chosen from the pattern width: the step size that leaves the fewest idle
+  // slots in the padded width, then the fewest steps (the best of the sizes
+  // compiled on each of the 27- and 7-point stencils and banded matrices of
+  // widths 5-41 measured).
   if constexpr (sizeof(V) == 8)
   {
     if (S.rpl == 1)
     {
-      if (S.wmax > 16)
+      const int u = sliced_step(wpad, {4, 8, 10, 14});
+      if (u == 14)
         go(integral_constant<int, 14>{}, integral_constant<int, 1>{});
-      else if (S.wmax > 4)
+      else if (u == 10)
+        go(integral_constant<int, 10>{}, integral_constant<int, 1>{});
+      else if (u == 8)
         go(integral_constant<int, 8>{}, integral_constant<int, 1>{});
       else
-        go(integral_constant<int, UNR>{}, integral_constant<int, 1>{});
+        go(integral_constant<int, 4>{}, integral_constant<int, 1>{});
       return nblk;
     }
   }
@@ -1153,7 +1178,7 @@ int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X*
   {
     if (S.rpl == 4)
     {
-      if (wpad % 4 == 0)
+      if (sliced_step(wpad, {2, 4}) == 4)
         go(integral_constant<int, 2>{}, integral_constant<int, 4>{});
       else
         go(integral_constant<int, 1>{}, integral_constant<int, 4>{});
@@ -1164,9 +1189,10 @@ int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X*
   {
     if (S.rpl == 2)
     {
-      if (wpad % 12 == 0)
+      const int u = sliced_step(wpad, {4, 8, 12});
+      if (u == 12)
         go(integral_constant<int, 3>{}, integral_constant<int, 2>{});
-      else if (wpad % 8 == 0)
+      else if (u == 8)
         go(integral_constant<int, 2>{}, integral_constant<int, 2>{});
       else
         go(integral_constant<int, 1>{}, integral_constant<int, 2>{});
