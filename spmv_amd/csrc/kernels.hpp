@@ -987,7 +987,9 @@ constexpr int sliced_vw()
 
 // One wavefront handles RPL consecutive slices (64 x RPL rows, lane l owns
 // row l of each) so that the value loads and gathers of all of them are in
-// flight together.
+// flight together; each step issues UNR value loads (VW entries each) per
+// slice before any product, and the launcher picks UNR from the slice width
+// (sliced_step).
 constexpr int kSliceBS = 256;
 template <typename V, typename X, typename Acc, int MODE, int UNR, int RPL>
 __global__ __launch_bounds__(kSliceBS) void sliced_spmv(
