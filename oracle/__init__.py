@@ -66,9 +66,15 @@ def lib():
         L.or_cg_f64.argtypes = [_i64, _i64, _i32p, _i32p, _f64p, _f64p, ctypes.c_int,
                                 ctypes.c_double, _vp, _f64p, _vp]
         L.or_cg_f64.restype = ctypes.c_int
+        L.or_cg_f64_mt.argtypes = [_i64, _i64, _i32p, _i32p, _f64p, _f64p, ctypes.c_int,
+                                   ctypes.c_double, _vp, _f64p, _vp, ctypes.c_int]
+        L.or_cg_f64_mt.restype = ctypes.c_int
         L.or_cg_f32.argtypes = [_i64, _i32p, _i32p, _f32p, _f32p, ctypes.c_int,
                                 ctypes.c_float, _f32p, _vp]
         L.or_cg_f32.restype = ctypes.c_int
+        L.or_cg_f32_restart.argtypes = [_i64, _i32p, _i32p, _f32p, _f32p, ctypes.c_int,
+                                        ctypes.c_float, _f32p, _vp, ctypes.c_int]
+        L.or_cg_f32_restart.restype = ctypes.c_int
         L.or_gen_demo1d.argtypes = [_i64, ctypes.c_double, _vp, _vp, _vp]
         L.or_gen_demo1d.restype = _i64
         L.or_gen_stencil.argtypes = [ctypes.c_int, _i64, _i64, _i64, _i64,
@@ -232,22 +238,35 @@ def l2g_reverse_update(local_size, num_ghosts, indexbuf, v, incoming):
     return out
 
 
-def cg(A, b, kmax, rtol, halo=None):
-    """spmv::cg restated; returns (x[ncols], k, hist[0..k] of squared norms)."""
+def cg(A, b, kmax, rtol, halo=None, threads=None):
+    """spmv::cg restated; returns (x[ncols], k, hist[0..k] of squared norms).
+    threads=None: one thread, sequential dot products.  threads=n: the product,
+    vector updates and dot products split over n OpenMP threads, the dot
+    products summed in a fixed 64-chunk order (result independent of n; it
+    differs from the sequential one only by the dot products' rounding)."""
     x = np.empty(A.ncols, np.float64)
     hist = np.zeros(kmax + 1, np.float64)
     halo = None if halo is None else np.ascontiguousarray(halo, np.float64)
-    k = lib().or_cg_f64(A.nrows, A.ncols, A.rowptr, A.colidx, A.val,
-                        np.ascontiguousarray(b, np.float64), kmax, rtol, _ptr(halo), x,
-                        _ptr(hist))
+    b = np.ascontiguousarray(b, np.float64)
+    if threads is not None:
+        k = lib().or_cg_f64_mt(A.nrows, A.ncols, A.rowptr, A.colidx, A.val, b, kmax, rtol,
+                               _ptr(halo), x, _ptr(hist), int(threads))
+    else:
+        k = lib().or_cg_f64(A.nrows, A.ncols, A.rowptr, A.colidx, A.val, b, kmax, rtol,
+                            _ptr(halo), x, _ptr(hist))
     return x, k, hist[: k + 1].copy()
 
 
-def cg_f32(A, val32, b, kmax, rtol):
+def cg_f32(A, val32, b, kmax, rtol, restarts=0):
+    """FP32 CG; restarts > 0 adds the true-residual check of the batched
+    solver (b - A x in double when the recursive residual meets rtol, restart
+    from x if the true one does not, at most `restarts` times)."""
     x = np.empty(A.nrows, np.float32)
     hist = np.zeros(kmax + 1, np.float32)
-    k = lib().or_cg_f32(A.nrows, A.rowptr, A.colidx, np.ascontiguousarray(val32, np.float32),
-                        np.ascontiguousarray(b, np.float32), kmax, rtol, x, _ptr(hist))
+    k = lib().or_cg_f32_restart(A.nrows, A.rowptr, A.colidx,
+                                np.ascontiguousarray(val32, np.float32),
+                                np.ascontiguousarray(b, np.float32), kmax, rtol, x, _ptr(hist),
+                                int(restarts))
     return x, k, hist[: k + 1].copy()
 
 

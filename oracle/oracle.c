@@ -234,6 +234,95 @@ static double or_dot(int64_t n, const double* a, const double* b)
   return s;
 }
 
+/* Dot product in a fixed chunked order: the range is cut into OR_DOT_CHUNKS
+ * contiguous chunks (the cut depends on n only), each chunk summed left to
+ * right, then the chunk sums left to right.  Deterministic whatever the
+ * number of threads; rounded differently from the sequential or_dot (the
+ * reference's own order is Eigen's SIMD reduction, which neither restates). */
+#define OR_DOT_CHUNKS 64
+static double or_dot_chunked(int64_t n, const double* a, const double* b, int nthreads)
+{
+  double part[OR_DOT_CHUNKS];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+  for (int c = 0; c < OR_DOT_CHUNKS; ++c)
+  {
+    const int64_t lo = n * c / OR_DOT_CHUNKS, hi = n * (c + 1) / OR_DOT_CHUNKS;
+    double s = 0.0;
+    for (int64_t i = lo; i < hi; ++i)
+      s += a[i] * b[i];
+    part[c] = s;
+  }
+  (void)nthreads;
+  double s = 0.0;
+  for (int c = 0; c < OR_DOT_CHUNKS; ++c)
+    s += part[c];
+  return s;
+}
+
+/* The same loop with its product, vector updates and dot products split over
+ * `nthreads` OpenMP threads, so that the checker can run CG on the
+ * full-size systems (10^8 entries, 10^3 iterations) in tens of seconds.
+ * Every row is still summed in stored order and every vector entry updated
+ * by the same expression as in or_cg_f64; only the dot products use the
+ * chunked order above.  The result does not depend on nthreads. */
+int or_cg_f64_mt(int64_t nrows, int64_t ncols, const int32_t* rowptr,
+                 const int32_t* colidx, const double* val, const double* b,
+                 int kmax, double rtol, const double* halo, double* x,
+                 double* hist, int nthreads)
+{
+  const int64_t M = nrows;
+  if (nthreads < 1)
+    nthreads = 1;
+  double* r = (double*)malloc(sizeof(double) * (size_t)M);
+  double* y = (double*)malloc(sizeof(double) * (size_t)M);
+  double* p = (double*)calloc((size_t)ncols, sizeof(double));
+  for (int64_t i = 0; i < ncols; ++i)
+    x[i] = 0.0;                                  /* cg.cpp:39 */
+  memcpy(r, b, sizeof(double) * (size_t)M);     /* cg.cpp:40 */
+  memcpy(p, r, sizeof(double) * (size_t)M);     /* cg.cpp:41 */
+  double rnorm0 = or_dot_chunked(M, r, r, nthreads); /* cg.cpp:42-44 */
+  if (hist)
+    hist[0] = rnorm0;
+  const double rtol2 = rtol * rtol;
+  double rnorm_old = rnorm0;
+  int k = 0;
+  while (k < kmax)
+  {
+    ++k;
+    for (int64_t g = M; g < ncols; ++g)         /* update(p), cg.cpp:55 */
+      p[g] = halo ? halo[g - M] : 0.0;
+    or_spmv_f64_omp(M, rowptr, colidx, val, p, y, nthreads); /* cg.cpp:56 */
+    double pdoty = or_dot_chunked(M, p, y, nthreads);        /* cg.cpp:59-61 */
+    double alpha = rnorm_old / pdoty;           /* cg.cpp:62 */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+    for (int64_t i = 0; i < M; ++i)
+    {
+      x[i] += alpha * p[i];                     /* cg.cpp:65 */
+      r[i] -= alpha * y[i];                     /* cg.cpp:66 */
+    }
+    double rnorm_new = or_dot_chunked(M, r, r, nthreads); /* cg.cpp:69-71 */
+    double beta = rnorm_new / rnorm_old;        /* cg.cpp:72 */
+    rnorm_old = rnorm_new;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+    for (int64_t i = 0; i < M; ++i)
+      p[i] = p[i] * beta + r[i];                /* cg.cpp:76 */
+    if (hist)
+      hist[k] = rnorm_new;
+    if (rnorm_new / rnorm0 < rtol2)             /* cg.cpp:78-79 */
+      break;
+  }
+  free(r);
+  free(y);
+  free(p);
+  return k;
+}
+
 int or_cg_f64(int64_t nrows, int64_t ncols, const int32_t* rowptr,
               const int32_t* colidx, const double* val, const double* b,
               int kmax, double rtol, const double* halo, double* x,
@@ -282,10 +371,16 @@ int or_cg_f64(int64_t nrows, int64_t ncols, const int32_t* rowptr,
 }
 
 /* FP32 CG (same loop, single precision throughout): checker for the FP32
- * inner solver and the batched solver. */
-int or_cg_f32(int64_t n, const int32_t* rowptr, const int32_t* colidx,
-              const float* val, const float* b, int kmax, float rtol,
-              float* x, float* hist)
+ * inner solver and the batched solver.  max_restarts > 0 adds the batched
+ * solver's true-residual check: when the recursive residual meets rtol, the
+ * true residual b - A x is formed with products and sums in double (the FP32
+ * values and x are exact in double); if it does not meet rtol either, CG
+ * restarts from x with r = p = (float)(b - A x) and ||r||^2 = (float) of its
+ * double norm, at most max_restarts times.  Iterations count on across
+ * restarts. */
+int or_cg_f32_restart(int64_t n, const int32_t* rowptr, const int32_t* colidx,
+                      const float* val, const float* b, int kmax, float rtol,
+                      float* x, float* hist, int max_restarts)
 {
   float* r = (float*)malloc(sizeof(float) * (size_t)n);
   float* y = (float*)malloc(sizeof(float) * (size_t)n);
@@ -297,13 +392,17 @@ int or_cg_f32(int64_t n, const int32_t* rowptr, const int32_t* colidx,
     p[i] = b[i];
   }
   float rnorm0 = 0.0f;
+  double bnorm2 = 0.0;
   for (int64_t i = 0; i < n; ++i)
+  {
     rnorm0 += r[i] * r[i];
+    bnorm2 += (double)b[i] * (double)b[i];
+  }
   if (hist)
     hist[0] = rnorm0;
   const float rtol2 = rtol * rtol;
   float rnorm_old = rnorm0;
-  int k = 0;
+  int k = 0, restarts = 0;
   while (k < kmax)
   {
     ++k;
@@ -326,12 +425,37 @@ int or_cg_f32(int64_t n, const int32_t* rowptr, const int32_t* colidx,
     if (hist)
       hist[k] = rnorm_new;
     if (rnorm_new / rnorm0 < rtol2)
-      break;
+    {
+      if (restarts >= max_restarts)
+        break;
+      double rt2 = 0.0;
+      for (int64_t i = 0; i < n; ++i)
+      {
+        double t = 0.0;
+        for (int32_t e = rowptr[i]; e < rowptr[i + 1]; ++e)
+          t += (double)val[e] * (double)x[colidx[e]];
+        const double ri = (double)b[i] - t;
+        rt2 += ri * ri;
+        r[i] = (float)ri;
+        p[i] = r[i];
+      }
+      if (rt2 < (double)rtol * (double)rtol * bnorm2)
+        break;
+      ++restarts;
+      rnorm_old = (float)rt2;
+    }
   }
   free(r);
   free(y);
   free(p);
   return k;
+}
+
+int or_cg_f32(int64_t n, const int32_t* rowptr, const int32_t* colidx,
+              const float* val, const float* b, int kmax, float rtol,
+              float* x, float* hist)
+{
+  return or_cg_f32_restart(n, rowptr, colidx, val, b, kmax, rtol, x, hist, 0);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -5,6 +5,8 @@ Tolerances (the reference's dot-product order is Eigen's SIMD order, which
 neither side reproduces): iteration count within +-1 and residual history
 max_k |rho_k^gpu - rho_k^oracle| <= 1e-10 with rho_k = sqrt(||r_k||^2/||b||^2)
 over the common iterations."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -263,17 +265,33 @@ def test_config_cpu_plain_poisson5_512():
     assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b) * 1.01
 
 
+# threads of the checker on the full-size systems (the GPU box gives a
+# process 16 cores)
+NT = max(1, min(16, os.cpu_count() or 1))
+
+
 def _true_rel(A, x, b):
-    r = O.spmv(_ocsr(A), x, threads=8) - b
+    r = O.spmv(_ocsr(A), x, threads=NT) - b
     return np.linalg.norm(r) / np.linalg.norm(b)
 
 
+def _check_history(k, hist, ko, histo, tol=1e-10):
+    """The CG contract: iteration count within one and the residual history
+    rho_k = sqrt(||r_k||^2 / ||b||^2) within tol over the common iterations."""
+    assert abs(k - ko) <= 1, (k, ko)
+    m = min(k, ko) + 1
+    d = np.max(np.abs(np.sqrt(hist[:m] / hist[0]) - np.sqrt(histo[:m] / histo[0])))
+    assert d <= tol, d
+
+
+@pytest.mark.timeout(600)
 def test_config_27pt_share_full_size_cg():
     """One rank's 160^3 share of the 27-point stencil with its ghost plane
     (109,443,836 entries), CG to 1e-10 with the neighbour's direction values
-    zero: the residual the solver reports and the true residual (oracle
-    product) both meet the tolerance, the ghost part of x is zero, and a
-    second solve repeats the first bit for bit."""
+    zero: iteration count within one and residual history within 1e-10 of
+    the oracle's CG on the same system (threaded checker), the true residual
+    meets the tolerance, the ghost part of x is zero, and a second solve
+    repeats the first bit for bit."""
     A = P.stencil(27, 160, 160, 160, ghost_planes=1)
     assert A.nnz == 109_443_836
     m = S.L2GMap(**P.slab_decomposition(160, 160, 160))
@@ -285,11 +303,19 @@ def test_config_27pt_share_full_size_cg():
     assert np.sqrt(hist[k] / hist[0]) < 1e-10
     xh = x.cpu().numpy()
     assert np.all(xh[A.nrows:] == 0.0)
-    assert _true_rel(A, xh, b) <= 1e-9
     x2, k2 = S.cg(M, bd, 5000, 1e-10)
     assert k2 == k and torch.equal(x2, x)
+    del M, x2
+    print(f"[27pt share] GPU CG {k} iterations; oracle CG on {NT} threads", flush=True)
+    xo, ko, histo = O.cg(_ocsr(A), b, 5000, 1e-10, threads=NT)
+    print(f"[27pt share] oracle CG {ko} iterations", flush=True)
+    _check_history(k, hist, ko, histo)
+    # both meet 1e-10 in residual; kappa ~ 3.5e3 bounds the difference by 7e-7
+    assert np.linalg.norm(xh - xo) <= 1e-7 * np.linalg.norm(xo)
+    assert _true_rel(A, xh, b) <= 1e-9
 
 
+@pytest.mark.timeout(900)
 def test_config_poisson256_full_size_cg_and_refinement():
     """The 256^3 7-point Poisson system (117,047,296 entries): FP64 CG and
     refinement with FP32 / BF16 values (CSR and sliced inner matrices) reach
@@ -300,8 +326,19 @@ def test_config_poisson256_full_size_cg_and_refinement():
     M64 = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
     b = P.uniform(5, A.nrows, -1, 1)
     bd = _dev(b)
-    x64, k = S.cg(M64, bd, 100000, 1e-10)
+    # the FP64 product at full size, bit for bit
+    xr = P.uniform(6, A.ncols, -1, 1)
+    assert np.array_equal(M64.mult(_dev(xr)).cpu().numpy(), O.spmv(_ocsr(A), xr, threads=NT))
+    x64, k, hist = S.cg(M64, bd, 100000, 1e-10, history=True)
     assert _true_rel(A, x64.cpu().numpy(), b) <= 2e-10
+    # the CG contract against the oracle's CG on the same system
+    print(f"[poisson256] GPU CG {k} iterations; oracle CG on {NT} threads", flush=True)
+    xo, ko, histo = O.cg(_ocsr(A), b, 100000, 1e-10, threads=NT)
+    print(f"[poisson256] oracle CG {ko} iterations", flush=True)
+    _check_history(k, hist, ko, histo)
+    # both meet 1e-10 in residual; kappa ~ 2.7e4 bounds the difference by 5.4e-6
+    assert np.linalg.norm(x64.cpu().numpy() - xo) <= 1e-7 * np.linalg.norm(xo)
+    del xo
     M64.set_kernel(S.SLICED_VARIANT)
     xs, ks = S.cg(M64, bd, 100000, 1e-10)
     assert abs(ks - k) <= 1
@@ -316,3 +353,23 @@ def test_config_poisson256_full_size_cg_and_refinement():
             assert _true_rel(A, x.cpu().numpy(), b) <= 1.01e-10
             assert torch.linalg.norm(x - x64).item() <= 1e-7 * n64
         del Ml
+
+
+def test_solvers_reject_null_vectors():
+    """NULL b or x is an argument error (SPMVX_ERR_ARG = 1) for every CG entry
+    point, raised on the host before any kernel is launched."""
+    import ctypes
+    A = P.stencil(7, 6, 6, 6)
+    M64 = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    M32 = M64.like("f32")
+    L = S.lib()
+    its = ctypes.c_int32(-7)
+    b = torch.ones(A.nrows, dtype=torch.float64, device=DEV)
+    bp = ctypes.c_void_p(b.data_ptr())
+    for bb, xx in ((None, bp), (bp, None)):
+        assert L.spmvx_cg(M64.handle, None, bb, xx, 10, 1e-10, ctypes.byref(its), None, None) == 1
+        assert L.spmvx_cg_f32(M32.handle, bb, xx, 10, 1e-6, ctypes.byref(its), None) == 1
+        assert L.spmvx_ir_cg(M64.handle, M32.handle, bb, xx, 4, 1e-10, 100, 1e-4, 0, None,
+                             None) == 1
+        assert b"NULL vector" in L.spmvx_last_error()
+    assert its.value == -7

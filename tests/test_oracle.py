@@ -265,6 +265,38 @@ def test_cg_ghost_halo():
     assert np.all(x0[A.nrows:] == 0.0)
 
 
+def test_cg_threaded_chunked_order():
+    """The threaded oracle CG (rows, vector updates and dot products split over
+    OpenMP threads; dots summed in a fixed 64-chunk order) gives the same
+    result for any thread count, converges in the same number of iterations
+    as the sequential oracle with its residual history within 1e-12 and its
+    solution within 1e-12 relative, with and without a halo, and keeps the
+    closed-form one-iteration convergence on an eigenvector."""
+    A = O.gen_stencil(27, 9, 9, 9, ghost_planes=1)
+    b = O.uniform(3, A.nrows, -1, 1)
+    halo = O.uniform(4, A.ncols - A.nrows, -1, 1)
+    for h in (None, halo):
+        x1, k1, h1 = O.cg(A, b, 500, 1e-10, halo=h)
+        runs = [O.cg(A, b, 500, 1e-10, halo=h, threads=t) for t in (1, 2, 5)]
+        for xt, kt, ht in runs:
+            assert kt == runs[0][1] and np.array_equal(xt, runs[0][0])
+            assert np.array_equal(ht, runs[0][2])
+        xt, kt, ht = runs[0]
+        assert kt == k1 and k1 > 5
+        assert np.max(np.abs(np.sqrt(ht / ht[0]) - np.sqrt(h1 / h1[0]))) <= 1e-12
+        assert np.linalg.norm(xt - x1) <= 1e-12 * np.linalg.norm(x1)
+    # a Dirichlet sine mode of the 7-point operator is an eigenvector: one
+    # iteration, exact solution v / lambda
+    n = 6
+    A7 = O.gen_stencil(7, n, n, n)
+    i = np.arange(n) + 1
+    s = np.sin(np.pi * i / (n + 1))
+    v = np.einsum("i,j,k->kji", s, s, s).ravel()
+    lam = 3 * 2 * (1 - np.cos(np.pi / (n + 1)))
+    x, k, _ = O.cg(A7, lam * v, 10, 1e-10, threads=3)
+    assert k == 1 and np.allclose(x, v, rtol=0, atol=1e-13)
+
+
 def test_cg_f32():
     A = O.gen_stencil(5, 16, 16)
     b = O.uniform(2, A.ncols, -1, 1)
