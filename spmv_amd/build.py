@@ -17,7 +17,7 @@ MPI_HOME = os.environ.get("MPI_HOME", "/opt/conda")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPMV_AMD_ARCH", "gfx950")
 
-SOURCES = ["capi.cpp", "petsc.cpp", "matrix.hip", "l2gmap.hip", "cg.hip", "batch.hip"]
+SOURCES = ["capi.cpp", "petsc.cpp", "matrix.hip", "transpose.hip", "l2gmap.hip", "cg.hip", "batch.hip"]
 
 # -ffp-contract=off: products and sums are rounded separately (the FP64
 # kernels are compared bit for bit with the reference's Eigen arithmetic).

@@ -130,7 +130,8 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
 
   const void* halo = col_map ? col_map->d_halo.ptr : nullptr;
   const int32_t halo_dt = col_map ? col_map->halo_dtype : -2;
-  if (!w.exec || w.g_x != x || w.g_hist != hist_dev || w.g_map != col_map || w.g_halo != halo
+  const uint64_t map_id = col_map ? col_map->id : 0;
+  if (!w.exec || w.g_x != x || w.g_hist != hist_dev || w.g_map != map_id || w.g_halo != halo
       || w.g_halo_dtype != halo_dt || w.g_variant != A->variant
       || w.g_long_rows != A->long_rows || w.g_mem != w.mem.ptr
       || w.g_values != vals)
@@ -149,7 +150,7 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
     SPMVX_HIP(hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0));
     w.g_x = x;
     w.g_hist = hist_dev;
-    w.g_map = col_map;
+    w.g_map = map_id;
     w.g_halo = halo;
     w.g_halo_dtype = halo_dt;
     w.g_variant = A->variant;
@@ -221,6 +222,25 @@ __global__ void add_correction(double* __restrict__ x, const float* __restrict__
     x[i] += norm * static_cast<double>(d[i]);
 }
 
+// Argument checks of spmvx_cg.
+void check_cg_args(spmvx_matrix A, spmvx_l2gmap col_map, const double* b, const double* x,
+                   int32_t max_its)
+{
+  SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+  SPMVX_REQUIRE(A->dtype == SPMVX_F64, "spmv::cg needs an FP64 matrix");
+  const Pattern& P = *A->pat;
+  if (col_map)
+  {
+    SPMVX_REQUIRE(col_map->local_size == P.nrows, "spmv::cg - Error: b.rows() != A.rows()");
+    SPMVX_REQUIRE(col_map->local_size + static_cast<int64_t>(col_map->ghosts.size()) == P.ncols,
+                  "column map size differs from the matrix column count");
+  }
+  else
+    SPMVX_REQUIRE(P.ncols == P.nrows, "a matrix with ghost columns needs its column map");
+  SPMVX_REQUIRE(max_its >= 0, "max_its must be >= 0");
+  SPMVX_REQUIRE((b && x) || P.nrows == 0, "NULL vector");
+}
+
 } // namespace
 } // namespace spmvx
 
@@ -232,20 +252,7 @@ int spmvx_cg(spmvx_matrix A, spmvx_l2gmap col_map, const double* b, double* x, i
              double rtol, int32_t* its, double* hist_host, spmvx_stream stream)
 {
   return guarded([&] {
-    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
-    SPMVX_REQUIRE(A->dtype == SPMVX_F64, "spmv::cg needs an FP64 matrix");
-    Pattern& P = *A->pat;
-    if (col_map)
-    {
-      SPMVX_REQUIRE(col_map->local_size == P.nrows,
-                    "spmv::cg - Error: b.rows() != A.rows()");
-      SPMVX_REQUIRE(col_map->local_size + static_cast<int64_t>(col_map->ghosts.size()) == P.ncols,
-                    "column map size differs from the matrix column count");
-    }
-    else
-      SPMVX_REQUIRE(P.ncols == P.nrows, "a matrix with ghost columns needs its column map");
-    SPMVX_REQUIRE(max_its >= 0, "max_its must be >= 0");
-    SPMVX_REQUIRE((b && x) || P.nrows == 0, "NULL vector");
+    check_cg_args(A, col_map, b, x, max_its);
     hipStream_t s = as_stream(stream);
     DevBuf hist;
     double* hd = nullptr;
@@ -273,6 +280,7 @@ int spmvx_cg_f32(spmvx_matrix A, const float* b, float* x, int32_t max_its, doub
     const Pattern& P = *A->pat;
     SPMVX_REQUIRE(P.ncols == P.nrows, "cg_f32 needs a square matrix without ghost columns");
     SPMVX_REQUIRE(max_its >= 0, "max_its must be >= 0");
+    SPMVX_REQUIRE((b && x) || P.nrows == 0, "NULL vector");
     hipStream_t s = as_stream(stream);
     int k;
     if (A->dtype == SPMVX_F32)
@@ -299,6 +307,8 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
     Pattern& P = *A64->pat;
     SPMVX_REQUIRE(P.nrows == P.ncols, "refinement needs a square matrix");
     SPMVX_REQUIRE(max_outer >= 0 && max_outer <= 64, "max_outer must be in [0, 64]");
+    SPMVX_REQUIRE((b && x) || P.nrows == 0, "NULL vector");
+    SPMVX_REQUIRE(inner_max_its >= 0, "inner_max_its must be >= 0");
     hipStream_t s = as_stream(stream);
     const int64_t n = P.nrows;
     const RowBlocks& rb64 = variant_blocks(P, A64->variant);

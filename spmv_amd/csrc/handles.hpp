@@ -118,6 +118,9 @@ struct Pattern
   std::map<std::pair<int, int>, std::unique_ptr<RowBlocks>> blocks; // by (chunk, rows)
   int device = 0;
   const RowBlocks& blocks_for(int chunk, int rows = kBS);
+  // the blocks for (chunk, rows) if they have been built, else null (never
+  // allocates)
+  const RowBlocks* find_blocks(int chunk, int rows);
   // sliced layout, analysed on first use; null with a reason if not eligible
   std::unique_ptr<RowPatterns> rpat;
   std::string rpat_error;
@@ -145,10 +148,38 @@ inline const RowBlocks& variant_blocks(Pattern& P, int var)
 {
   return P.blocks_for(variant_block_nnz(var), variant_block_rows(var));
 }
+inline const RowBlocks* built_variant_blocks(Pattern& P, int var)
+{
+  return P.find_blocks(variant_block_nnz(var), variant_block_rows(var));
+}
 
 // Builds the sliced layout of a matrix (row patterns, slices, values) if it
 // is missing; throws with the reason when the pattern is not eligible.
 SlicedView sliced_view(spmvx_matrix_s* A);
+
+// Bytes of one stored value of a matrix dtype.
+inline size_t dtype_size(int32_t dtype)
+{
+  switch (dtype)
+  {
+  case SPMVX_F64: return 8;
+  case SPMVX_F32: return 4;
+  case SPMVX_BF16: return 2;
+  case SPMVX_C64: return 16;
+  case SPMVX_C32: return 8;
+  default: throw Error(SPMVX_ERR_ARG, "unknown dtype");
+  }
+}
+
+// Transposed copy of A (transpose.hip), built on stream s, which the call
+// synchronises; and the re-gather of its values after a value update
+// (enqueued on s).
+void build_transpose(spmvx_matrix_s* A, hipStream_t s);
+void gather_transposed_values(spmvx_matrix_s* A, hipStream_t s);
+
+// Waits for the last value update of A (spmvx_matrix_set_values runs on the
+// caller's stream) before work that reads the values outside that stream.
+void wait_values(spmvx_matrix_s* A);
 
 // Records the launches of f on stream q (capture mode thread-local) as a
 // graph.  If f throws, the capture is ended and its partial graph dropped
@@ -212,7 +243,7 @@ struct CgWorkspace
   hipGraphExec_t exec = nullptr;
   const void* g_x = nullptr;   // arguments the graph was captured with
   const void* g_hist = nullptr;
-  const void* g_map = nullptr;
+  uint64_t g_map = 0; // id of the column map (0: none)
   const void* g_halo = nullptr;
   int32_t g_halo_dtype = -2;
   int32_t g_variant = -1;
@@ -244,19 +275,33 @@ struct spmvx_matrix_s
   int32_t long_rows = 0; // 1: rows longer than a block are summed in parallel
   std::shared_ptr<spmvx::Pattern> pat;
   spmvx::DevBuf values;
-  // transposed copy for spmv_t (built on first use)
+  // transposed copy for spmv_t (built on first use): tvalues[d] = values[tperm[d]]
   std::shared_ptr<spmvx::Pattern> tpat;
-  spmvx::DevBuf tvalues;
+  spmvx::DevBuf tvalues, tperm;
+  // recorded by spmvx_matrix_set_values on the caller's stream
+  hipEvent_t values_event = nullptr;
+  bool values_pending = false;
   // values in the sliced row-pattern layout (built when that kernel is chosen)
   spmvx::DevBuf svalues;
   bool svalues_ready = false;
   // solver scratch (FP64 / FP32)
   spmvx::CgWorkspace ws64, ws32;
   spmvx::PinnedBuf host_flag;
+  spmvx_matrix_s() = default;
+  spmvx_matrix_s(const spmvx_matrix_s&) = delete;
+  spmvx_matrix_s& operator=(const spmvx_matrix_s&) = delete;
+  ~spmvx_matrix_s()
+  {
+    if (values_event)
+      (void)hipEventDestroy(values_event);
+  }
 };
 
 struct spmvx_l2gmap_s
 {
+  // unique for the process (never reused, unlike the handle's address): keys
+  // what a captured CG graph recorded from this map
+  uint64_t id = 0;
   int32_t nranks = 1, rank = 0;
   std::vector<int64_t> ranges;
   std::vector<int64_t> ghosts;

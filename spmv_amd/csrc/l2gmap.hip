@@ -3,6 +3,7 @@
 // (:126-170): device pack of the owned entries the neighbours ghost, device
 // unpack of the halo into the contiguous ghost region.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 
 #include <type_traits>
@@ -150,6 +151,8 @@ int spmvx_l2gmap_create(int32_t nranks, int32_t rank, const int64_t* local_sizes
     SPMVX_REQUIRE(local_sizes != nullptr, "local_sizes is NULL");
     SPMVX_REQUIRE(nghosts >= 0 && (nghosts == 0 || ghosts != nullptr), "invalid ghosts");
     auto m = std::make_unique<spmvx_l2gmap_s>();
+    static std::atomic<uint64_t> next_id{1};
+    m->id = next_id.fetch_add(1);
     m->nranks = nranks;
     m->rank = rank;
     // ranges: Allgather of local sizes + prefix sum (L2GMap.cpp:23-28)

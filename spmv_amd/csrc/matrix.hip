@@ -55,6 +55,13 @@ const RowBlocks& Pattern::blocks_for(int chunk, int rows)
   return ref;
 }
 
+const RowBlocks* Pattern::find_blocks(int chunk, int rows)
+{
+  std::lock_guard<std::recursive_mutex> lock(lazy_mu);
+  auto it = blocks.find(std::pair<int, int>(chunk, rows));
+  return it == blocks.end() ? nullptr : it->second.get();
+}
+
 const RowPatterns* Pattern::row_patterns()
 {
   std::lock_guard<std::recursive_mutex> lock(lazy_mu);
@@ -255,18 +262,6 @@ __device__ __forceinline__ uint16_t dev_f64_to_bf16(double d)
 
 bool is_complex(int32_t dtype) { return dtype == SPMVX_C64 || dtype == SPMVX_C32; }
 
-size_t dtype_size(int32_t dtype)
-{
-  switch (dtype)
-  {
-  case SPMVX_F64: return 8;
-  case SPMVX_F32: return 4;
-  case SPMVX_BF16: return 2;
-  case SPMVX_C64: return 16;
-  case SPMVX_C32: return 8;
-  default: throw Error(SPMVX_ERR_ARG, "unknown dtype");
-  }
-}
 
 __global__ void convert_values(const double* __restrict__ src, void* __restrict__ dst,
                                int64_t n, int32_t dtype)
@@ -525,40 +520,16 @@ void validate_csr(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t* rowp
     SPMVX_REQUIRE(colidx[k] >= 0 && colidx[k] < ncols, "column index out of range");
 }
 
-void build_transpose(spmvx_matrix_s* A)
-{
-  const Pattern& P = *A->pat;
-  const size_t es = dtype_size(A->dtype);
-  std::vector<int32_t> rp(P.nrows + 1), ci(P.nnz);
-  std::vector<uint8_t> v(es * P.nnz);
-  SPMVX_HIP(hipMemcpy(rp.data(), P.rowptr.ptr, sizeof(int32_t) * (P.nrows + 1), hipMemcpyDeviceToHost));
-  if (P.nnz)
-  {
-    SPMVX_HIP(hipMemcpy(ci.data(), P.colidx.ptr, sizeof(int32_t) * P.nnz, hipMemcpyDeviceToHost));
-    SPMVX_HIP(hipMemcpy(v.data(), A->values.ptr, es * P.nnz, hipMemcpyDeviceToHost));
-  }
-  // counting sort by column; rows ascending within each column
-  std::vector<int32_t> trp(P.ncols + 1, 0), tci(P.nnz);
-  std::vector<uint8_t> tv(es * P.nnz);
-  for (int64_t k = 0; k < P.nnz; ++k)
-    ++trp[ci[k] + 1];
-  for (int64_t c = 0; c < P.ncols; ++c)
-    trp[c + 1] += trp[c];
-  std::vector<int32_t> next(trp.begin(), trp.end() - 1);
-  for (int64_t r = 0; r < P.nrows; ++r)
-    for (int32_t k = rp[r]; k < rp[r + 1]; ++k)
-    {
-      const int32_t dst = next[ci[k]]++;
-      tci[dst] = static_cast<int32_t>(r);
-      std::memcpy(&tv[es * dst], &v[es * k], es);
-    }
-  A->tpat = upload_pattern(P.ncols, P.nrows, P.nnz, trp.data(), tci.data());
-  A->tvalues.ensure(es * static_cast<size_t>(P.nnz + 16));
-  if (P.nnz)
-    SPMVX_HIP(hipMemcpy(A->tvalues.ptr, tv.data(), es * P.nnz, hipMemcpyHostToDevice));
-}
-
 } // namespace
+
+void wait_values(spmvx_matrix_s* A)
+{
+  if (A->values_pending)
+  {
+    SPMVX_HIP(hipEventSynchronize(A->values_event));
+    A->values_pending = false;
+  }
+}
 
 SlicedView sliced_view(spmvx_matrix_s* A)
 {
@@ -570,6 +541,7 @@ SlicedView sliced_view(spmvx_matrix_s* A)
   const SliceLayout& L = P.slice_layout(vw);
   if (!A->svalues_ready)
   {
+    wait_values(A); // the scatter below runs on the null stream
     A->svalues.release();
     A->svalues.ensure(dtype_size(A->dtype) * static_cast<size_t>(L.slots + 16));
     scatter_values(A, *R, L, nullptr);
@@ -646,6 +618,7 @@ int spmvx_matrix_create(int32_t dtype, int64_t nrows, int64_t ncols, int64_t nnz
     A->dtype = dtype;
     A->variant = default_variant(dtype);
     A->pat = upload_pattern(nrows, ncols, nnz, rowptr, colidx);
+    (void)variant_blocks(*A->pat, A->variant);
     A->values.ensure(es * static_cast<size_t>(nnz + 16));
     if (nnz > 0)
     {
@@ -680,8 +653,10 @@ int spmvx_matrix_create_like(spmvx_matrix base, int32_t dtype, spmvx_matrix* out
     A->variant = default_variant(dtype);
     A->long_rows = base->long_rows;
     A->pat = base->pat;
+    (void)variant_blocks(*A->pat, A->variant);
     const int64_t nnz = base->pat->nnz;
     A->values.ensure(es * static_cast<size_t>(nnz + 16));
+    wait_values(base); // the conversion below runs on the null stream
     if (nnz > 0)
     {
       convert_values<<<1024, 256>>>(base->values.as<double>(), A->values.ptr, nnz, dtype);
@@ -736,9 +711,13 @@ int spmvx_matrix_set_values(spmvx_matrix A, const double* values_dev, spmvx_stre
         const int vw = static_cast<int>(8 / dtype_size(A->dtype));
         scatter_values(A, *A->pat->row_patterns(), A->pat->slice_layout(vw), as_stream(stream));
       }
+      if (A->tpat) // the transposed copy keeps its pattern: values re-gathered
+        gather_transposed_values(A, as_stream(stream));
+      if (!A->values_event)
+        SPMVX_HIP(hipEventCreateWithFlags(&A->values_event, hipEventDisableTiming));
+      SPMVX_HIP(hipEventRecord(A->values_event, as_stream(stream)));
+      A->values_pending = true;
     }
-    A->tpat.reset(); // transposed copy is stale
-    A->tvalues.release();
   });
 }
 
@@ -749,9 +728,11 @@ int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant)
     SPMVX_REQUIRE(variant >= -1 && variant <= kSlicedVariant, "unknown kernel variant");
     SPMVX_REQUIRE(!is_complex(A->dtype) || variant <= 4,
                   "kernel variant not available for complex values (use 0-4)");
-    if (variant == kSlicedVariant)
+    const int v = variant < 0 ? default_variant(A->dtype) : variant;
+    if (v == kSlicedVariant)
       (void)sliced_view(A); // builds the layout or reports why it does not apply
-    A->variant = variant < 0 ? default_variant(A->dtype) : variant;
+    (void)variant_blocks(*A->pat, v); // row blocks built here, not at the first product
+    A->variant = v;
   });
 }
 
@@ -840,7 +821,7 @@ int spmvx_spmv_t(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
     SPMVX_REQUIRE(A->pat->nnz == 0 || x != nullptr, "x is NULL");
     require_disjoint(x, A->pat->nrows, y, A->pat->ncols, vec_bytes(A->dtype));
     if (!A->tpat)
-      build_transpose(A);
+      build_transpose(A, as_stream(stream));
     launch_spmv(A->dtype, csr_variant(A), *A->tpat, A->tvalues.ptr, x, y, as_stream(stream),
                 A->long_rows);
   });

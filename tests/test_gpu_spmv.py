@@ -72,6 +72,65 @@ def test_spmv_transpose_bitexact(name, A):
     assert np.array_equal(yt, O.spmv_t(_ocsr(A), x))
 
 
+def _rect_matrix(nrows=3000, ncols=4500, seed=31):
+    """Non-symmetric rectangular matrix with empty rows, empty columns (the
+    last 500 are never referenced) and rows of 0-60 random entries."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(nrows):
+        k = 0 if i % 11 == 0 else int(rng.integers(1, 60))
+        rows.append(np.sort(rng.choice(ncols - 500, k, replace=False)).astype(np.int32))
+    rowptr = np.zeros(nrows + 1, np.int64)
+    rowptr[1:] = np.cumsum([len(r) for r in rows])
+    colidx = np.concatenate(rows)
+    return P.CSR(nrows, ncols, rowptr.astype(np.int32), colidx,
+                 P.uniform(seed, len(colidx), -1, 1))
+
+
+@pytest.mark.parametrize("dtype", ["f64", "c64"])
+def test_transpose_rectangular_and_value_updates_on_side_stream(dtype):
+    """The transposed copy is built on the device (stable sort by column) on
+    the caller's stream: bit-exact against the oracle's transposed product on
+    a non-symmetric rectangular matrix with empty rows and columns, and after
+    value updates made on a side stream (non-blocking, as torch's are) and
+    followed by transposed products on that same stream: first before the
+    copy exists, then with the copy kept and its values re-gathered."""
+    A = _rect_matrix()
+    oc = _ocsr(A)
+    x = P.uniform(41, A.nrows, -1, 1)
+    side = torch.cuda.Stream()
+    if dtype == "f64":
+        M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+        assert np.array_equal(M.transpmult(_dev(x)).cpu().numpy(), O.spmv_t(oc, x))
+        for seed in (42, 43):
+            v2 = P.uniform(seed, A.nnz, -3, 3)
+            v2d = _dev(v2)
+            xd = _dev(x)
+            torch.cuda.synchronize()
+            with torch.cuda.stream(side):
+                M.set_values(v2d)
+                yt = M.transpmult(xd)
+            side.synchronize()
+            B = P.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, v2)
+            assert np.array_equal(yt.cpu().numpy(), O.spmv_t(_ocsr(B), x))
+            assert np.array_equal(M.mult(_dev(P.uniform(seed, A.ncols, -1, 1))).cpu().numpy(),
+                                  O.spmv(_ocsr(B), P.uniform(seed, A.ncols, -1, 1)))
+    else:
+        v = A.val + 1j * P.uniform(44, A.nnz, -1, 1)
+        M = S.Matrix(A.rowptr, A.colidx, v, ncols=A.ncols, dtype="c64")
+        xc = x + 1j * P.uniform(45, A.nrows, -1, 1)
+        assert np.array_equal(M.transpmult(_cdev(xc)).cpu().numpy(), O.spmv_t_complex(oc, v, xc))
+        v2 = P.uniform(46, A.nnz, -1, 1) - 1j * P.uniform(47, A.nnz, -1, 1)
+        v2d = _cdev(v2).view(torch.float64)
+        xd = _cdev(xc)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(side):
+            M.set_values(v2d)
+            yt = M.transpmult(xd)
+        side.synchronize()
+        assert np.array_equal(yt.cpu().numpy(), O.spmv_t_complex(oc, v2, xc))
+
+
 def _bf16_round(v):
     """double -> bf16 (round to nearest even) -> float32, on the host."""
     f = v.astype(np.float32)
