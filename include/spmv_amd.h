@@ -342,9 +342,14 @@ int spmvx_batch_set_config(spmvx_batch B, int32_t block, int32_t lanes_per_row,
  * ||r_k||^2 / ||b||^2 < rtol^2 or max_its, in one persistent kernel launch
  * (workgroups take team slots, largest systems first, from a device work
  * queue; team members synchronise through bounded waits), replayed from a
- * hipGraph captured on the first call with these arguments.  its_host (may be
- * NULL): nsys*nrhs iteration counts; when given the stream is synchronised and
- * a timed-out team wait is reported as SPMVX_ERR_STATE.
+ * hipGraph captured on the first call with these arguments.  When the
+ * recursive residual meets rtol, the true residual b - A x is formed in
+ * double; a right-hand side whose true residual does not meet rtol restarts
+ * CG from its x (r = p = b - A x), at most 3 times per solve, so the returned
+ * x meets ||b - A x|| <= rtol ||b|| and not only the FP32 recursion.
+ * its_host (may be NULL): nsys*nrhs iteration counts (counted on across
+ * restarts).  The stream is synchronised; a timed-out team wait is reported
+ * as SPMVX_ERR_STATE (results invalid).
  */
 int spmvx_batch_solve(spmvx_batch B, const float* b_dev, float* x_dev, int32_t max_its,
                       double rtol, int32_t* its_host, spmvx_stream stream);

@@ -15,9 +15,12 @@
 //   * G lanes cooperate on a row (rows per wavefront = 64/G), chosen per
 //     system; G and the block size are picked by the autotuner.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "handles.hpp"
 
@@ -86,6 +89,85 @@ __device__ __forceinline__ VecN<NRHS> block_sum_n(VecN<NRHS> a, float (*red)[16]
   for (int j = 0; j < NRHS; ++j)
     a.v[j] = wave_sum(lane < nw ? red[j][lane] : 0.0f);
   return a;
+}
+
+template <int NRHS>
+struct VecD
+{
+  double v[NRHS];
+};
+
+// Sum over the block of NRHS doubles per thread; result in all threads.
+template <int NRHS>
+__device__ __forceinline__ VecD<NRHS> block_sum_d(VecD<NRHS> a, double (*red)[16])
+{
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NRHS; ++j)
+    a.v[j] = wave_sum(a.v[j]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < NRHS; ++j)
+      red[j][wid] = a.v[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NRHS; ++j)
+    a.v[j] = wave_sum(lane < nw ? red[j][lane] : 0.0);
+  return a;
+}
+
+// True residual over rows [r0, r1), G lanes per row: t = b - A x with the
+// caller's b read through perm (when the batch is reordered), the products
+// and sums in double (the FP32 values and x are exact in double),
+// r = p = (float) t; returns this thread's share of ||t||^2 (group leaders).
+template <int NRHS, int G>
+__device__ __forceinline__ VecD<NRHS> batch_resid(const int32_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ colidx,
+                                                  const float* __restrict__ val,
+                                                  const int32_t* __restrict__ perm,
+                                                  const float* __restrict__ b,
+                                                  const float* __restrict__ x,
+                                                  float* __restrict__ r, float* __restrict__ p,
+                                                  int32_t r0, int32_t r1)
+{
+  VecD<NRHS> tt{};
+  const int gl = threadIdx.x % G;
+  const int rows_per_pass = blockDim.x / G;
+  for (int32_t row = r0 + static_cast<int32_t>(threadIdx.x / G); row < r1; row += rows_per_pass)
+  {
+    VecD<NRHS> acc{};
+    const int32_t e0 = rowptr[row], e1 = rowptr[row + 1];
+    for (int32_t e = e0 + gl; e < e1; e += G)
+    {
+      const double a = static_cast<double>(val[e]);
+      const VecN<NRHS> xc = ldn<NRHS>(x + static_cast<int64_t>(colidx[e]) * NRHS);
+#pragma unroll
+      for (int j = 0; j < NRHS; ++j)
+        acc.v[j] += a * static_cast<double>(xc.v[j]);
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int j = 0; j < NRHS; ++j)
+        acc.v[j] += __shfl_xor(acc.v[j], off, 64);
+    if (gl == 0)
+    {
+      const int64_t o = static_cast<int64_t>(row) * NRHS;
+      const VecN<NRHS> bi = ldn<NRHS>(b + static_cast<int64_t>(perm ? perm[row] : row) * NRHS);
+      VecN<NRHS> ri;
+#pragma unroll
+      for (int j = 0; j < NRHS; ++j)
+      {
+        const double t = static_cast<double>(bi.v[j]) - acc.v[j];
+        tt.v[j] += t * t;
+        ri.v[j] = static_cast<float>(t);
+      }
+      stn<NRHS>(r + o, ri);
+      stn<NRHS>(p + o, ri);
+    }
+  }
+  return tt;
 }
 
 // y = A p over rows [r0, r1) with G lanes per row; returns this thread's
@@ -202,6 +284,29 @@ __device__ __forceinline__ VecN<NRHS> team_sum(VecN<NRHS> v, float* tred, int wh
   return t;
 }
 
+// team_sum in double (buffers of W * NRHS doubles, two alternating).
+template <int NRHS>
+__device__ __forceinline__ VecD<NRHS> team_sum_d(VecD<NRHS> v, double* tredd, int which, int W,
+                                                 int m, unsigned* bar, unsigned& nbar, int* err,
+                                                 double (*red)[16])
+{
+  v = block_sum_d<NRHS>(v, red);
+  if (W == 1)
+    return v;
+  double* buf = tredd + static_cast<int64_t>(which) * W * NRHS;
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < NRHS; ++j)
+      buf[m * NRHS + j] = v.v[j];
+  team_barrier(bar, static_cast<unsigned>(W) * (++nbar), err);
+  VecD<NRHS> t{};
+  for (int q = 0; q < W; ++q)
+#pragma unroll
+    for (int j = 0; j < NRHS; ++j)
+      t.v[j] += buf[q * NRHS + j];
+  return t;
+}
+
 // Index of the XCD (accelerator complex die) the calling wave runs on, from
 // the XCC_ID hardware register.  Used only to pick a work queue: a wrong
 // value costs locality, never correctness.
@@ -211,19 +316,24 @@ __device__ __forceinline__ int current_xcd()
   return static_cast<int>(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20)) & 0xF;
 }
 
+// True-residual restarts per solve (each costs one product in double).
+constexpr int kMaxRestarts = 3;
+
 template <int NRHS>
 __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
     const float* __restrict__ val, const int32_t* __restrict__ sys_row,
     const int2* __restrict__ slots, const int32_t* __restrict__ qoff, int32_t nq,
     const int32_t* __restrict__ team, const int64_t* __restrict__ tred_off,
-    const int8_t* __restrict__ gsel, const float* __restrict__ b, float* __restrict__ x,
+    const int8_t* __restrict__ gsel, const int32_t* __restrict__ perm,
+    const float* __restrict__ b, float* __restrict__ xu, float* __restrict__ xin,
     float* __restrict__ r, float* __restrict__ p, float* __restrict__ y, int32_t max_its,
-    float rtol2, int32_t* __restrict__ its_out, unsigned* queue, unsigned* bars,
+    float rtol2, double rtol2d, int32_t* __restrict__ its_out, unsigned* queue, unsigned* bars,
     float* tred_all, int* err)
 {
   __shared__ int2 job;
   __shared__ float red[NRHS][16];
+  __shared__ double redd[NRHS][16];
   // queues are per XCD: members of one team share an L2; a workgroup whose
   // own queue is empty takes slots from the next ones
   const int home = current_xcd() % nq;
@@ -261,14 +371,19 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
     const int g = gsel[s];
     unsigned* bar = bars + s;
     float* tred = tred_all + tred_off[s];
+    // double sums after the float ones (an even float offset: 8-byte aligned)
+    double* tredd = reinterpret_cast<double*>(tred + 2 * static_cast<int64_t>(W) * NRHS);
     unsigned nbar = 0;
+    int nsum_d = 0;
 
-    // x = 0, r = b, p = b, rr0 = ||b||^2 (cg.cpp:31-44)
+    // x = 0, r = b, p = b, rr0 = ||b||^2 (cg.cpp:31-44); with a reordered
+    // batch, row i of the solver is row perm[i] of b and x
+    float* __restrict__ x = perm ? xin : xu;
     VecN<NRHS> rr{};
     for (int32_t i = lo + static_cast<int32_t>(threadIdx.x); i < hi; i += blockDim.x)
     {
       const int64_t o = static_cast<int64_t>(i) * NRHS;
-      const VecN<NRHS> bi = ldn<NRHS>(b + o);
+      const VecN<NRHS> bi = ldn<NRHS>(b + static_cast<int64_t>(perm ? perm[i] : i) * NRHS);
       stn<NRHS>(x + o, VecN<NRHS>{});
       stn<NRHS>(r + o, bi);
       stn<NRHS>(p + o, bi);
@@ -276,15 +391,33 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
       for (int j = 0; j < NRHS; ++j)
         rr.v[j] += bi.v[j] * bi.v[j];
     }
+    // ||b||^2 in double as well, for the true-residual check
+    VecD<NRHS> bb{};
+#pragma unroll
+    for (int j = 0; j < NRHS; ++j)
+      bb.v[j] = static_cast<double>(rr.v[j]);
     rr = team_sum<NRHS>(rr, tred, 0, W, m, bar, nbar, err, red); // also publishes p
+    {
+      VecD<NRHS> b2{};
+      for (int32_t i = lo + static_cast<int32_t>(threadIdx.x); i < hi; i += blockDim.x)
+      {
+        const VecN<NRHS> bi = ldn<NRHS>(b + static_cast<int64_t>(perm ? perm[i] : i) * NRHS);
+#pragma unroll
+        for (int j = 0; j < NRHS; ++j)
+          b2.v[j] += static_cast<double>(bi.v[j]) * static_cast<double>(bi.v[j]);
+      }
+      bb = team_sum_d<NRHS>(b2, tredd, (nsum_d++) & 1, W, m, bar, nbar, err, redd);
+    }
     const VecN<NRHS> rr0 = rr;
-    bool active[NRHS];
+    bool active[NRHS], by_rtol[NRHS];
     int32_t its[NRHS];
     bool any = false;
+    int restarts = 0;
 #pragma unroll
     for (int j = 0; j < NRHS; ++j)
     {
       active[j] = max_its > 0;
+      by_rtol[j] = false;
       its[j] = 0;
       any = any || active[j];
     }
@@ -333,7 +466,12 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
         {
           its[j] = k;
           rr.v[j] = rn.v[j];
-          if (rn.v[j] / rr0.v[j] < rtol2 || !(rn.v[j] == rn.v[j]))
+          if (rn.v[j] / rr0.v[j] < rtol2)
+          {
+            active[j] = false;
+            by_rtol[j] = true;
+          }
+          else if (!(rn.v[j] == rn.v[j]))
             active[j] = false;
         }
         any = any || active[j];
@@ -351,7 +489,42 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
             xi.v[j] = xi.v[j] + alpha[j] * pi.v[j];
           stn<NRHS>(x + o, xi);
         }
-        break;
+        if (restarts >= kMaxRestarts || k == max_its)
+          break;
+        // True-residual check of the right-hand sides that met rtol by the
+        // recursive residual: t = b - A x in double; where ||t|| does not
+        // meet rtol either, CG restarts from x with r = p = t (the recursive
+        // residual of FP32 CG drifts from the true one by ~1e-7 relative).
+        if (W > 1)
+          team_barrier(bar, static_cast<unsigned>(W) * (++nbar), err); // x complete
+        else
+          __syncthreads();
+        VecD<NRHS> tt;
+        switch (g)
+        {
+        case 1: tt = batch_resid<NRHS, 1>(rowptr, colidx, val, perm, b, x, r, p, lo, hi); break;
+        case 2: tt = batch_resid<NRHS, 2>(rowptr, colidx, val, perm, b, x, r, p, lo, hi); break;
+        case 4: tt = batch_resid<NRHS, 4>(rowptr, colidx, val, perm, b, x, r, p, lo, hi); break;
+        case 8: tt = batch_resid<NRHS, 8>(rowptr, colidx, val, perm, b, x, r, p, lo, hi); break;
+        default: tt = batch_resid<NRHS, 16>(rowptr, colidx, val, perm, b, x, r, p, lo, hi); break;
+        }
+        // (also publishes r and p to the team)
+        tt = team_sum_d<NRHS>(tt, tredd, (nsum_d++) & 1, W, m, bar, nbar, err, redd);
+        ++restarts;
+#pragma unroll
+        for (int j = 0; j < NRHS; ++j)
+        {
+          if (by_rtol[j] && !(tt.v[j] < rtol2d * bb.v[j]))
+          {
+            active[j] = true;
+            rr.v[j] = static_cast<float>(tt.v[j]);
+            any = true;
+          }
+          by_rtol[j] = false;
+        }
+        if (!any)
+          break;
+        continue; // the next iteration's product reads p = r
       }
       for (int32_t i = lo + static_cast<int32_t>(threadIdx.x); i < hi; i += blockDim.x)
       {
@@ -369,6 +542,14 @@ __global__ __launch_bounds__(kMaxBatchBS) void batch_cg_kernel(
       }
       if (W > 1)
         team_barrier(bar, static_cast<unsigned>(W) * (++nbar), err); // p complete
+    }
+    if (perm)
+    {
+      // this member's rows of x back to the caller's order
+      __syncthreads();
+      for (int32_t i = lo + static_cast<int32_t>(threadIdx.x); i < hi; i += blockDim.x)
+        stn<NRHS>(xu + static_cast<int64_t>(perm[i]) * NRHS,
+                  ldn<NRHS>(x + static_cast<int64_t>(i) * NRHS));
     }
     if (m == 0 && threadIdx.x == 0)
 #pragma unroll
@@ -409,6 +590,9 @@ struct spmvx_batch_s
   int64_t nrows = 0, nnz = 0;
   std::vector<int32_t> h_sys_row, h_rowptr;
   DevBuf rowptr, colidx, val, sys_row, gsel, scratch, its, queue;
+  // reverse Cuthill-McKee row order of every system (new row -> caller's row)
+  DevBuf perm;
+  bool reordered = false;
   // team plan
   DevBuf slots, qoff, team, tred_off, bars, tred, err;
   int32_t nslots = 0;
@@ -425,7 +609,7 @@ struct spmvx_batch_s
   const void* cap_b = nullptr;
   void* cap_x = nullptr;
   int32_t cap_its = -1;
-  float cap_rtol2 = -1.0f;
+  double cap_rtol = -1.0;
   // the batch's own stream (graphs cannot be captured on the null stream),
   // ordered after / before the caller's stream with events
   hipStream_t own = nullptr;
@@ -447,6 +631,91 @@ struct spmvx_batch_s
 
 namespace
 {
+
+// Reverse Cuthill-McKee order of the rows of one system (George & Liu):
+// breadth-first from a pseudo-peripheral row, the unvisited neighbours of
+// each row taken in increasing degree, every connected component in turn,
+// the whole order reversed.  rp/ci describe the system's rows with columns
+// local to the system.  Rows that share columns end up close together, so
+// the direction-vector gathers of neighbouring rows share cache lines.
+std::vector<int32_t> rcm_order(int32_t n, const int32_t* rp, const int32_t* ci)
+{
+  std::vector<int32_t> order;
+  order.reserve(static_cast<size_t>(n));
+  std::vector<int32_t> stamp(static_cast<size_t>(n), -1), vis, nb;
+  std::vector<char> done(static_cast<size_t>(n), 0);
+  int32_t cur = 0;
+  auto deg = [&](int32_t u) { return rp[u + 1] - rp[u]; };
+  // breadth-first levels from s; returns the depth and the last level's start
+  auto bfs = [&](int32_t s0, size_t& last) {
+    ++cur;
+    vis.assign(1, s0);
+    stamp[s0] = cur;
+    size_t begin = 0, end = 1;
+    int depth = 0;
+    for (;;)
+    {
+      for (size_t h = begin; h < end; ++h)
+        for (int32_t k = rp[vis[h]]; k < rp[vis[h] + 1]; ++k)
+          if (stamp[ci[k]] != cur)
+          {
+            stamp[ci[k]] = cur;
+            vis.push_back(ci[k]);
+          }
+      if (vis.size() == end)
+      {
+        last = begin;
+        return depth;
+      }
+      begin = end;
+      end = vis.size();
+      ++depth;
+    }
+  };
+  for (int32_t start = 0; start < n; ++start)
+  {
+    if (done[start])
+      continue;
+    // pseudo-peripheral row of this component
+    int32_t s0 = start;
+    size_t last = 0;
+    int d = bfs(s0, last);
+    for (int t = 0; t < 4; ++t)
+    {
+      int32_t best = vis[last];
+      for (size_t h = last; h < vis.size(); ++h)
+        if (deg(vis[h]) < deg(best))
+          best = vis[h];
+      size_t last2 = 0;
+      const int d2 = bfs(best, last2);
+      if (d2 <= d)
+        break;
+      s0 = best;
+      d = d2;
+      last = last2;
+    }
+    // Cuthill-McKee from s0
+    size_t head = order.size();
+    order.push_back(s0);
+    done[s0] = 1;
+    while (head < order.size())
+    {
+      const int32_t u = order[head++];
+      nb.clear();
+      for (int32_t k = rp[u]; k < rp[u + 1]; ++k)
+        if (!done[ci[k]])
+        {
+          done[ci[k]] = 1;
+          nb.push_back(ci[k]);
+        }
+      std::stable_sort(nb.begin(), nb.end(),
+                       [&](int32_t a, int32_t c) { return deg(a) < deg(c); });
+      order.insert(order.end(), nb.begin(), nb.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
 
 int batch_capacity(spmvx_batch_s* B)
 {
@@ -517,7 +786,7 @@ void build_plan(spmvx_batch_s* B, int capacity)
   }
   std::vector<int64_t> off(nsys + 1, 0);
   for (int s = 0; s < nsys; ++s)
-    off[s + 1] = off[s] + 2LL * team[s] * B->nrhs;
+    off[s + 1] = off[s] + 6LL * team[s] * B->nrhs; // 2 float + 2 double buffers
   B->nslots = static_cast<int32_t>(slots.size());
   B->gsel.ensure(std::max(nsys, 1));
   B->team.ensure(4 * std::max(nsys, 1));
@@ -538,13 +807,15 @@ void build_plan(spmvx_batch_s* B, int capacity)
   B->plan_dirty = false;
 }
 
-void launch_batch(spmvx_batch_s* B, const float* b, float* x, int32_t max_its, float rtol2,
+void launch_batch(spmvx_batch_s* B, const float* b, float* x, int32_t max_its, double rtol,
                   hipStream_t s)
 {
   const int64_t vn = B->nrows * B->nrhs;
   float* r = B->scratch.as<float>();
   float* p = r + vn;
   float* y = p + vn;
+  float* xin = y + vn; // x in the solver's row order (reordered batches)
+  const int32_t* perm = B->reordered ? B->perm.as<int32_t>() : nullptr;
   batch_reset<<<std::max(1, std::min(64, (B->nsys + 255) / 256)), 256, 0, s>>>(
       B->queue.as<unsigned>(), B->nq, B->bars.as<unsigned>(), B->nsys, B->err.as<int>());
 #define SPMVX_BATCH(NR)                                                                        \
@@ -552,7 +823,8 @@ void launch_batch(spmvx_batch_s* B, const float* b, float* x, int32_t max_its, f
       B->rowptr.as<int32_t>(), B->colidx.as<int32_t>(), B->val.as<float>(),                    \
       B->sys_row.as<int32_t>(), B->slots.as<int2>(), B->qoff.as<int32_t>(), B->nq,            \
       B->team.as<int32_t>(),                                                                   \
-      B->tred_off.as<int64_t>(), B->gsel.as<int8_t>(), b, x, r, p, y, max_its, rtol2,          \
+      B->tred_off.as<int64_t>(), B->gsel.as<int8_t>(), perm, b, x, xin, r, p, y, max_its,      \
+      static_cast<float>(rtol * rtol), rtol * rtol,                                            \
       B->its.as<int32_t>(), B->queue.as<unsigned>(), B->bars.as<unsigned>(),                   \
       B->tred.as<float>(), B->err.as<int>())
   if (B->nrhs == 4)
@@ -607,6 +879,81 @@ int spmvx_batch_create(int32_t nsys, const int32_t* sys_row, const int32_t* rowp
     B->nrows = nrows;
     B->nnz = nnz;
     B->h_sys_row.assign(sys_row, sys_row + nsys + 1);
+    // every system reordered by reverse Cuthill-McKee (SPMVX_BATCH_REORDER=0
+    // keeps the caller's order): rows, columns and values permuted here, b
+    // and x read and written through the permutation by the solver
+    const char* env = std::getenv("SPMVX_BATCH_REORDER");
+    B->reordered = !(env && std::atoi(env) == 0) && nrows > 0;
+    std::vector<int32_t> nrp, nci, perm;
+    std::vector<float> nva;
+    if (B->reordered)
+    {
+      nrp.assign(static_cast<size_t>(nrows) + 1, 0);
+      nci.resize(static_cast<size_t>(nnz));
+      nva.resize(static_cast<size_t>(nnz));
+      perm.resize(static_cast<size_t>(nrows));
+      std::vector<int32_t> inv(static_cast<size_t>(nrows));
+      auto work = [&](int s) {
+        const int32_t base = sys_row[s], n = sys_row[s + 1] - base;
+        std::vector<int32_t> lrp(static_cast<size_t>(n) + 1), lci;
+        const int32_t e0 = rowptr[base];
+        lci.resize(static_cast<size_t>(rowptr[base + n] - e0));
+        for (int32_t i = 0; i <= n; ++i)
+          lrp[i] = rowptr[base + i] - e0;
+        for (size_t k = 0; k < lci.size(); ++k)
+          lci[k] = colidx[e0 + k] - base;
+        const std::vector<int32_t> ord = rcm_order(n, lrp.data(), lci.data());
+        for (int32_t i = 0; i < n; ++i)
+        {
+          perm[base + i] = base + ord[i];
+          inv[base + ord[i]] = base + i;
+        }
+      };
+      const int nth = std::max(1, std::min<int>(16, static_cast<int>(std::thread::hardware_concurrency())));
+      {
+        std::vector<std::thread> pool;
+        std::atomic<int> next{0};
+        for (int t = 0; t < nth; ++t)
+          pool.emplace_back([&] {
+            for (int s; (s = next.fetch_add(1)) < nsys;)
+              work(s);
+          });
+        for (auto& th : pool)
+          th.join();
+      }
+      for (int64_t i = 0; i < nrows; ++i)
+        nrp[i + 1] = nrp[i] + (rowptr[perm[i] + 1] - rowptr[perm[i]]);
+      auto fill = [&](int64_t i0, int64_t i1) {
+        std::vector<std::pair<int32_t, float>> row;
+        for (int64_t i = i0; i < i1; ++i)
+        {
+          row.clear();
+          for (int32_t k = rowptr[perm[i]]; k < rowptr[perm[i] + 1]; ++k)
+            row.emplace_back(inv[colidx[k]], values[k]);
+          std::sort(row.begin(), row.end(),
+                    [](const std::pair<int32_t, float>& a, const std::pair<int32_t, float>& c) {
+                      return a.first < c.first;
+                    });
+          for (size_t q = 0; q < row.size(); ++q)
+          {
+            nci[nrp[i] + q] = row[q].first;
+            nva[nrp[i] + q] = row[q].second;
+          }
+        }
+      };
+      {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth; ++t)
+          pool.emplace_back(fill, nrows * t / nth, nrows * (t + 1) / nth);
+        for (auto& th : pool)
+          th.join();
+      }
+      rowptr = nrp.data();
+      colidx = nci.data();
+      values = nva.data();
+      B->perm.ensure(4 * static_cast<size_t>(nrows));
+      SPMVX_HIP(hipMemcpy(B->perm.ptr, perm.data(), 4 * nrows, hipMemcpyHostToDevice));
+    }
     B->h_rowptr.assign(rowptr, rowptr + nrows + 1);
     B->rowptr.ensure(4 * (nrows + 1));
     SPMVX_HIP(hipMemcpy(B->rowptr.ptr, rowptr, 4 * (nrows + 1), hipMemcpyHostToDevice));
@@ -619,7 +966,8 @@ int spmvx_batch_create(int32_t nsys, const int32_t* sys_row, const int32_t* rowp
     }
     B->sys_row.ensure(4 * (nsys + 1));
     SPMVX_HIP(hipMemcpy(B->sys_row.ptr, sys_row, 4 * (nsys + 1), hipMemcpyHostToDevice));
-    B->scratch.ensure(sizeof(float) * 3 * static_cast<size_t>(std::max<int64_t>(nrows, 1)) * nrhs);
+    // r, p, y and x in the solver's row order
+    B->scratch.ensure(sizeof(float) * 4 * static_cast<size_t>(std::max<int64_t>(nrows, 1)) * nrhs);
     B->its.ensure(4 * static_cast<size_t>(std::max(nsys, 1)) * nrhs);
     B->queue.ensure(256);
     *out = B.release();
@@ -670,7 +1018,6 @@ int spmvx_batch_solve(spmvx_batch B, const float* b, float* x, int32_t max_its, 
       B->grid = std::max(1, std::min(B->nslots, cap));
       drop_graph(B);
     }
-    const float rtol2 = static_cast<float>(rtol * rtol);
     if (!B->own)
     {
       SPMVX_HIP(hipStreamCreateWithFlags(&B->own, hipStreamNonBlocking));
@@ -678,31 +1025,31 @@ int spmvx_batch_solve(spmvx_batch B, const float* b, float* x, int32_t max_its, 
       SPMVX_HIP(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming));
     }
     if (!B->exec || B->cap_b != b || B->cap_x != x || B->cap_its != max_its
-        || B->cap_rtol2 != rtol2)
+        || B->cap_rtol != rtol)
     {
       drop_graph(B);
-      B->graph = capture_graph(B->own, [&] { launch_batch(B, b, x, max_its, rtol2, B->own); });
+      B->graph = capture_graph(B->own, [&] { launch_batch(B, b, x, max_its, rtol, B->own); });
       SPMVX_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
       B->cap_b = b;
       B->cap_x = x;
       B->cap_its = max_its;
-      B->cap_rtol2 = rtol2;
+      B->cap_rtol = rtol;
     }
     SPMVX_HIP(hipEventRecord(B->ev_in, s));
     SPMVX_HIP(hipStreamWaitEvent(B->own, B->ev_in, 0));
     SPMVX_HIP(hipGraphLaunch(B->exec, B->own));
     SPMVX_HIP(hipEventRecord(B->ev_out, B->own));
     SPMVX_HIP(hipStreamWaitEvent(s, B->ev_out, 0));
+    // the failure flag is always read (a timed-out team wait leaves invalid
+    // results): the call synchronises the stream
+    int err = 0;
     if (its_host)
-    {
-      int err = 0;
       SPMVX_HIP(hipMemcpyAsync(its_host, B->its.ptr, 4 * static_cast<size_t>(B->nsys) * B->nrhs,
                                hipMemcpyDeviceToHost, s));
-      SPMVX_HIP(hipMemcpyAsync(&err, B->err.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
-      SPMVX_HIP(hipStreamSynchronize(s));
-      if (err)
-        throw Error(SPMVX_ERR_STATE, "batched CG: a team barrier timed out (results invalid)");
-    }
+    SPMVX_HIP(hipMemcpyAsync(&err, B->err.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPMVX_HIP(hipStreamSynchronize(s));
+    if (err)
+      throw Error(SPMVX_ERR_STATE, "batched CG: a team barrier timed out (results invalid)");
   });
 }
 

@@ -1,6 +1,9 @@
 """GPU checks of the batched persistent CG (extension 2) against the oracle's
-FP32 CG (the loop of spmv/cg.cpp:50-80 in single precision), per system and
-right-hand side."""
+FP32 CG (the loop of spmv/cg.cpp:50-80 in single precision, with the batched
+solver's true-residual check: when the recursive residual meets rtol, b - A x
+is formed in double and CG restarts from x if it does not), per system and
+right-hand side.  Every right-hand side must reach a true residual
+||b - A x|| / ||b|| <= rtol = 1e-6, computed on the host in double."""
 import numpy as np
 import pytest
 import torch
@@ -22,8 +25,14 @@ def rhs(nrows, nrhs, seed):
     return P.uniform(seed, nrows * nrhs, -1, 1).astype(np.float32).reshape(nrows, nrhs)
 
 
+@pytest.mark.parametrize("reorder", [False, True], ids=["caller_order", "rcm"])
 @pytest.mark.parametrize("nrhs,rpm", [(4, 0), (4, 700), (2, 1000), (1, 16384)])
-def test_batch_matches_oracle(nrhs, rpm):
+def test_batch_matches_oracle(nrhs, rpm, reorder, monkeypatch):
+    """Without reordering the solver sums in another order than the oracle
+    (iterations within 3); with the default reverse Cuthill-McKee order the
+    system is the same matrix with permuted rows and columns, whose FP32 CG
+    rounds differently again: iterations within 3 or 5%."""
+    monkeypatch.setenv("SPMVX_BATCH_REORDER", "1" if reorder else "0")
     systems = make_systems([1500, 700, 3000, 2200, 999, 4100, 1800, 640], 100)
     B = S.Batch(systems, nrhs=nrhs)
     B.set_config(1024, 0, rpm)
@@ -34,17 +43,16 @@ def test_batch_matches_oracle(nrhs, rpm):
         lo, hi = B.sys_row[s], B.sys_row[s + 1]
         oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
         for j in range(nrhs):
-            xo, ko, _ = O.cg_f32(oc, A.val.astype(np.float32), b[lo:hi, j], 2000, 1e-6)
-            assert abs(int(its[s, j]) - ko) <= 2, (s, j, its[s, j], ko)
+            xo, ko, _ = O.cg_f32(oc, A.val.astype(np.float32), b[lo:hi, j], 2000, 1e-6,
+                                 restarts=3)
+            # the reduction orders differ, and so may whether a restart is
+            # needed (each costs one or two iterations)
+            tol = max(3, int(0.05 * ko)) if reorder else 3
+            assert abs(int(its[s, j]) - ko) <= tol, (s, j, its[s, j], ko)
             bn = np.linalg.norm(b[lo:hi, j])
             r = b[lo:hi, j].astype(np.float64) - O.spmv(oc, xh[lo:hi, j].astype(np.float64))
-            assert np.linalg.norm(r) <= 5e-6 * bn
+            assert np.linalg.norm(r) <= 1e-6 * bn
             assert np.linalg.norm(xh[lo:hi, j] - xo) <= 1e-4 * np.linalg.norm(xo)
-            # the recursive residual stops below rtol; the true residual of an
-            # FP32 CG sits near 1e-6 (FP32 rounding of x and r), as the
-            # oracle's own FP32 CG shows: no worse than it, or below rtol
-            ro = b[lo:hi, j].astype(np.float64) - O.spmv(oc, xo.astype(np.float64))
-            assert np.linalg.norm(r) <= max(1.25 * np.linalg.norm(ro), 1e-6 * bn)
 
 
 def true_rel_residuals(systems, B, b, x):
@@ -59,9 +67,9 @@ def true_rel_residuals(systems, B, b, x):
 
 
 def test_batch_every_config_and_replay():
-    """Every (workgroup size, lanes per row) pair converges in the same
-    number of iterations (+-2; the reduction order differs) to a solution
-    whose true residual meets the tolerance."""
+    """Every (workgroup size, lanes per row) pair converges in about the same
+    number of iterations (within 3 or 5%: the reduction order differs) to a
+    solution whose true residual meets the tolerance."""
     systems = make_systems([900, 2600, 1300, 300, 4000], 300)
     B = S.Batch(systems, nrhs=4)
     bh = rhs(B.nrows, 4, 3)
@@ -72,8 +80,8 @@ def test_batch_every_config_and_replay():
             for rpm in (0, 300, 1000):  # 0: one workgroup per system; else teams
                 B.set_config(bs, g, rpm)
                 x, its = B.solve(bd, max_its=1000, rtol=1e-6)
-                assert np.max(np.abs(its - its0)) <= 2
-                assert np.max(true_rel_residuals(systems, B, bh, x.cpu().numpy())) <= 5e-6
+                assert np.max(np.abs(its - its0)) <= max(3, int(0.05 * its0.max()))
+                assert np.max(true_rel_residuals(systems, B, bh, x.cpu().numpy())) <= 1e-6
     # graph replays: same answer every time, with eager work in between
     B.set_config(256, 0, 500)
     out = torch.empty_like(bd)
@@ -116,7 +124,7 @@ def test_config_batch_full_size():
     """The full batch: 256 SPD systems, rows log-uniform in [1e3, 2e5] (9.9M
     rows), 4 right-hand sides each, one persistent launch to 1e-6; every
     right-hand side converges, and the true residuals of every 32nd system
-    meet the FP32 accuracy bound."""
+    meet the tolerance."""
     nsys, nrhs = 256, 4
     ns = np.exp(np.log(1e3) + (np.log(2e5) - np.log(1e3)) * P.uniform(17, nsys)).astype(np.int64)
     systems = [P.random_spd(int(n), 1000 + i) for i, n in enumerate(ns)]
@@ -132,4 +140,4 @@ def test_config_batch_full_size():
         oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
         for j in range(nrhs):
             r = bh[lo:hi, j].astype(np.float64) - O.spmv(oc, xh[lo:hi, j].astype(np.float64))
-            assert np.linalg.norm(r) <= 5e-6 * np.linalg.norm(bh[lo:hi, j]), (s, j)
+            assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm(bh[lo:hi, j]), (s, j)
