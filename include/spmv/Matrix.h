@@ -18,6 +18,14 @@ class Matrix
 public:
   /// Copies the CSR in (Matrix.cpp:13-22) and runs the row-binning analysis.
   Matrix(const CSR<T>& A, std::shared_ptr<L2GMap> col_map, std::shared_ptr<L2GMap> row_map);
+#ifdef SPMV_HAVE_EIGEN
+  /// The reference's constructor (Matrix.h:30-32).
+  Matrix(const Eigen::SparseMatrix<T, Eigen::RowMajor>& A, std::shared_ptr<L2GMap> col_map,
+         std::shared_ptr<L2GMap> row_map)
+      : Matrix(CSR<T>(A), std::move(col_map), std::move(row_map))
+  {
+  }
+#endif
   ~Matrix();
   Matrix(Matrix&& o) noexcept;
   Matrix(const Matrix&) = delete;
@@ -33,6 +41,9 @@ public:
   std::int64_t rows() const { return _rows; }
   std::int64_t cols() const { return _cols; }
   std::int64_t nonZeros() const { return _nnz; }
+  /// Host copy of the stored matrix (Matrix.h:52-53 returns the Eigen
+  /// matrix; here it lives in device memory, so this copies it back).
+  CSR<T> mat() const;
 
   spmvx_matrix handle() const { return _h; }
 

@@ -10,6 +10,16 @@
 #include <cstdint>
 #include <vector>
 
+// Eigen adaptor: where Eigen is installed, the containers also convert from
+// and to the reference's Eigen types (plain element copies).
+#if defined(__has_include)
+#if __has_include(<Eigen/Sparse>)
+#include <Eigen/Dense>
+#include <Eigen/Sparse>
+#define SPMV_HAVE_EIGEN 1
+#endif
+#endif
+
 namespace spmv
 {
 
@@ -23,6 +33,32 @@ struct CSR
   std::vector<std::int32_t> inner; ///< column index per entry
   std::vector<T> values;           ///< value per entry
   std::int64_t nonZeros() const { return outer.empty() ? 0 : outer.back(); }
+
+#ifdef SPMV_HAVE_EIGEN
+  CSR() = default;
+  /// From the reference's input type (Matrix.h:30-32): the entries of every
+  /// row in stored order (compressed or not).
+  explicit CSR(const Eigen::SparseMatrix<T, Eigen::RowMajor>& A) : rows(A.rows()), cols(A.cols())
+  {
+    outer.assign(static_cast<std::size_t>(rows) + 1, 0);
+    for (Eigen::Index r = 0; r < A.outerSize(); ++r)
+    {
+      for (typename Eigen::SparseMatrix<T, Eigen::RowMajor>::InnerIterator it(A, r); it; ++it)
+      {
+        inner.push_back(static_cast<std::int32_t>(it.col()));
+        values.push_back(it.value());
+      }
+      outer[static_cast<std::size_t>(r) + 1] = static_cast<std::int32_t>(inner.size());
+    }
+  }
+  /// The same matrix as Eigen stores it (what Matrix::mat() returns in the
+  /// reference, Matrix.h:52-53).
+  Eigen::SparseMatrix<T, Eigen::RowMajor> to_eigen() const
+  {
+    return Eigen::Map<const Eigen::SparseMatrix<T, Eigen::RowMajor>>(
+        rows, cols, nonZeros(), outer.data(), inner.data(), values.data());
+  }
+#endif
 };
 
 /// Vector in device memory (owning, movable, not copyable implicitly).
@@ -40,6 +76,19 @@ public:
   Vector& operator=(const Vector&) = delete;
   /// Deep copy on the device.
   Vector clone() const;
+#ifdef SPMV_HAVE_EIGEN
+  /// From / to the reference's vector type (Eigen::VectorXd for T = double).
+  explicit Vector(const Eigen::Matrix<T, Eigen::Dynamic, 1>& host)
+      : Vector(std::vector<T>(host.data(), host.data() + host.size()))
+  {
+  }
+  Eigen::Matrix<T, Eigen::Dynamic, 1> to_eigen() const
+  {
+    const std::vector<T> h = to_host();
+    return Eigen::Map<const Eigen::Matrix<T, Eigen::Dynamic, 1>>(
+        h.data(), static_cast<Eigen::Index>(h.size()));
+  }
+#endif
 
   std::size_t size() const { return _n; }
   std::size_t rows() const { return _n; }

@@ -15,4 +15,16 @@ namespace spmv
 std::tuple<Vector<double>, int> cg(MPI_Comm comm, const Matrix<double>& A,
                                    const Vector<double>& b, int max_its, double rtol);
 
+#ifdef SPMV_HAVE_EIGEN
+/// The reference's signature (cg.h:27-29): b and x in host memory.
+inline std::tuple<Eigen::VectorXd, int> cg(MPI_Comm comm, const Matrix<double>& A,
+                                           const Eigen::Ref<const Eigen::VectorXd>& b,
+                                           int max_its, double rtol)
+{
+  const Vector<double> bd(std::vector<double>(b.data(), b.data() + b.size()));
+  auto [x, k] = cg(comm, A, bd, max_its, rtol);
+  return {x.to_eigen(), k};
+}
+#endif
+
 } // namespace spmv

@@ -330,6 +330,32 @@ Vector<T> Matrix<T>::transpmult(const Vector<T>& b) const
   return y;
 }
 
+template <typename T>
+CSR<T> Matrix<T>::mat() const
+{
+  CSR<T> A;
+  A.rows = _rows;
+  A.cols = _cols;
+  A.outer.resize(static_cast<std::size_t>(_rows) + 1);
+  A.inner.resize(static_cast<std::size_t>(_nnz));
+  A.values.resize(static_cast<std::size_t>(_nnz));
+  const std::int32_t *rp = nullptr, *ci = nullptr;
+  const void* va = nullptr;
+  check(spmvx_matrix_arrays(_h, &rp, &ci, &va));
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  hip_check(hipMemcpy(A.outer.data(), rp, sizeof(std::int32_t) * A.outer.size(),
+                      hipMemcpyDeviceToHost), "hipMemcpy");
+  if (_nnz > 0)
+  {
+    hip_check(hipMemcpy(A.inner.data(), ci, sizeof(std::int32_t) * A.inner.size(),
+                        hipMemcpyDeviceToHost), "hipMemcpy");
+    // the stored values have T's layout (F64, F32, C64, C32)
+    hip_check(hipMemcpy(A.values.data(), va, sizeof(T) * A.values.size(), hipMemcpyDeviceToHost),
+              "hipMemcpy");
+  }
+  return A;
+}
+
 template class Matrix<double>;
 template class Matrix<float>;
 template class Matrix<std::complex<double>>; // Matrix.cpp:441-444

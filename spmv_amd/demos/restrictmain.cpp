@@ -57,6 +57,14 @@ static int run(const std::string& rfile, const std::string& bfile, T rscale, T q
   auto cmap = std::const_pointer_cast<spmv::L2GMap>(l2g);
   auto rmap = std::const_pointer_cast<spmv::L2GMap>(R0.row_map());
   spmv::Matrix<T> R(A, cmap, rmap);
+  {
+    // Matrix::mat() (Matrix.h:52-53): the stored matrix read back equals the
+    // one handed in
+    const spmv::CSR<T> back = R.mat();
+    const bool same = back.rows == A.rows && back.cols == A.cols && back.outer == A.outer
+                      && back.inner == A.inner && back.values == A.values;
+    std::printf("mat.equal = %d\n", same ? 1 : 0);
+  }
   std::vector<T> qh(q0.size());
   for (std::size_t i = 0; i < q0.size(); ++i)
     qh[i] = T(q0[i]) * qscale;

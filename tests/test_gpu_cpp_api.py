@@ -80,5 +80,6 @@ def test_restrictmain_petsc(tmp_path, mode):
         pn = float(np.sum(np.abs(p) ** 2))
         q = R @ p
         qn = float(np.sum(np.abs(q) ** 2))
+    assert field(out, "mat.equal") == 1
     assert abs(field(out, "pnorm") - pn) <= 1e-12 * pn
     assert abs(field(out, "qnorm") - qn) <= 1e-12 * qn
