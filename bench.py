@@ -11,10 +11,15 @@ value = algorithmic SpMV bytes per step / measured step time (GB/s), with
     B = nnz*(8+4) + 4*(M+1) + 8*ncols + 8*M   (x read once, y written once).
 
 Also reported: the SpMV kernel's roofline (HIP events on its stream), the
-streaming-read and copy ceilings measured on the same GPU in the same run,
-the CPU oracle timed on the host (1 thread, bounded sample), and the CG
-time-to-1e-10 of FP64 CG versus mixed-precision refinement (FP32 and BF16
-values) on the 256^3 7-point Poisson system.
+update and product phases timed apart (as demos/matvecmain.cpp:72-80 does),
+the streaming-read and copy ceilings measured on the same GPU in the same
+run, the CPU oracle timed on the host (1 thread, bounded sample, and all
+cores), CG to 1e-10 on the share (GPU, and the oracle on all cores), the
+reference's own CPU configuration (5-point 512^2: 100 applications of the
+matvec loop and CG to 1e-8, GPU and oracle on 1 thread and all cores), the
+CG time-to-1e-10 of FP64 CG versus mixed-precision refinement (FP32 and BF16
+values) on the 256^3 7-point Poisson system (with the oracle's CG on all
+cores), the batched solver and the adjoint.
 """
 import argparse
 import json
@@ -49,7 +54,91 @@ def timed_loop(torch, fn, iters):
     return e0.elapsed_time(e1) / iters
 
 
-def run_cg_section(torch, S, P, O, n, seed, cpu_iters):
+def cpu_threads():
+    """Host threads for the all-cores CPU figures (the process's CPU share)."""
+    n = os.cpu_count() or 1
+    try:
+        n = min(n, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def run_c1_section(torch, S, P, O, do_cpu, nthreads, n=512, napply=100, rtol=1e-8):
+    """The reference's own CPU configuration: 5-point Poisson on n x n (one
+    rank, no ghosts), x uniform[-1,1] (seed 11).  The matvec loop of
+    demos/matvecmain.cpp:70-86 (update, y = A x, x[:M] = y) napply times,
+    then CG (cg.cpp:11-83) with b = A x_true (x_true uniform[-1,1], seed 12),
+    x0 = 0, to rtol; on the GPU and on the host (the oracle: 1 thread, and
+    all cores), on the same inputs."""
+    import numpy as np
+    dev = "cuda:0"
+    A = P.stencil(5, n, n)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    l2g = S.L2GMap(A.nrows)
+    x0 = P.uniform(11, A.ncols, -1, 1)
+    Bs = spmv_bytes(A.nnz, A.nrows, A.ncols)
+    out = {"workload": f"5-point Poisson {n}^2 ({A.nrows} rows, {A.nnz} nnz), FP64, one rank",
+           "matvec_loop": f"{napply} x (L2GMap::update, y = A x, x[:M] = y)", "cg_rtol": rtol}
+    x = torch.from_numpy(x0).to(dev)
+    y = torch.empty(A.nrows, dtype=torch.float64, device=dev)
+
+    def step():
+        l2g.update(x)
+        M.mult(x, out=y)
+        x[: A.nrows].copy_(y)
+
+    for _ in range(3):
+        step()
+    x.copy_(torch.from_numpy(x0))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(napply):
+        step()
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t
+    xg = x.cpu().numpy()
+    out["gpu_matvec_loop_seconds"] = tg
+    out["gpu_ms_per_apply"] = 1e3 * tg / napply
+    out["spmv_bytes"] = Bs
+    xt = P.uniform(12, A.ncols, -1, 1)
+    oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
+    b = O.spmv(oc, xt)
+    bd = torch.from_numpy(b).to(dev)
+    S.cg(M, bd, 2, rtol)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    xs, k = S.cg(M, bd, 100000, rtol)
+    torch.cuda.synchronize()
+    tcg = time.perf_counter() - t
+    xsh = xs.cpu().numpy()
+    out["gpu_cg"] = {"seconds": tcg, "iterations": k,
+                     "true_rel_residual": float(np.linalg.norm(O.spmv(oc, xsh) - b) / np.linalg.norm(b))}
+    log(f"[c1] GPU: {napply} applications {tg * 1e3:.2f} ms, CG {tcg:.4f}s {k} its")
+    if do_cpu:
+        for th in (1, nthreads):
+            xc = x0.copy()
+            t = time.perf_counter()
+            for _ in range(napply):
+                xc[: A.nrows] = O.spmv(oc, xc, threads=th)
+            tc = time.perf_counter() - t
+            t = time.perf_counter()
+            xo, ko, _ = O.cg(oc, b, 100000, rtol, threads=None if th == 1 else th)
+            tco = time.perf_counter() - t
+            key = "cpu_1_thread" if th == 1 else "cpu_all_cores"
+            out[key] = {"threads": th, "matvec_loop_seconds": tc, "ms_per_apply": 1e3 * tc / napply,
+                        "matvec_result_bit_identical_to_gpu": bool(np.array_equal(xc, xg)),
+                        "cg_seconds": tco, "cg_iterations": ko,
+                        "cg_oracle": "sequential dots" if th == 1 else "threaded, 64-chunk dots",
+                        "gpu_speedup_matvec": tc / tg, "gpu_speedup_cg": tco / tcg}
+            log(f"[c1] CPU {th} thread(s): {napply} applications {tc:.3f}s, CG {tco:.3f}s {ko} its")
+    return out
+
+
+def run_cg_section(torch, S, P, O, n, seed, cpu_iters, nthreads=1):
     """FP64 CG vs mixed-precision refinement to 1e-10 on the n^3 7-point
     Poisson system (b uniform[-1,1])."""
     dev = "cuda:0"
@@ -59,6 +148,7 @@ def run_cg_section(torch, S, P, O, n, seed, cpu_iters):
     M64 = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
     M32 = M64.like("f32")
     M16 = M64.like("bf16")
+    import numpy as np
     b = torch.from_numpy(P.uniform(seed, A.nrows, -1, 1)).to(dev)
     bnorm = torch.linalg.norm(b).item()
 
@@ -121,12 +211,24 @@ def run_cg_section(torch, S, P, O, n, seed, cpu_iters):
     if cpu_iters > 0:
         oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
         bh = b.cpu().numpy()
+        # all cores: the whole solve to 1e-10 (threaded oracle)
+        t = time.perf_counter()
+        xo, ko, _ = O.cg(oc, bh, 100000, 1e-10, threads=nthreads)
+        tc = time.perf_counter() - t
+        out["cpu_all_cores"] = {"threads": nthreads, "seconds": tc, "iterations": ko,
+                                "rel_diff_vs_gpu_fp64": float(np.linalg.norm(xo - x64.cpu().numpy())
+                                                              / np.linalg.norm(xo)),
+                                "gpu_fp64_speedup": tc / t64}
+        log(f"[cg] CPU {nthreads} threads: {tc:.2f}s {ko} its")
+        # one thread: an estimate from a sample of iterations (a whole solve
+        # would take minutes)
         t = time.perf_counter()
         O.cg(oc, bh, cpu_iters, 1e-30)
-        tc = time.perf_counter() - t
-        out["cpu_fp64_ms_per_iteration"] = 1e3 * tc / cpu_iters
-        out["cpu_fp64_estimated_seconds"] = tc / cpu_iters * k
-        out["cpu_sample"] = f"{cpu_iters} oracle CG iterations, 1 thread"
+        tc1 = time.perf_counter() - t
+        out["cpu_1_thread_estimate"] = {"ms_per_iteration": 1e3 * tc1 / cpu_iters,
+                                        "estimated_seconds": tc1 / cpu_iters * ko,
+                                        "sample": f"{cpu_iters} oracle CG iterations, 1 thread, "
+                                                  "scaled to the all-cores iteration count"}
     del M16, M32, M64
     return out
 
@@ -246,12 +348,16 @@ def main():
     ap.add_argument("--cg-n", type=int, default=256, help="grid edge of the CG comparison")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--fd-entries", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--mode", choices=["bench", "spmv", "copy"], default="bench",
-                    help="spmv: only the SpMV kernel loop; copy: the calibration copy kernel")
+    ap.add_argument("--mode", choices=["bench", "spmv", "copy", "read"], default="bench",
+                    help="spmv: only the SpMV kernel loop; copy: the calibration copy kernel; "
+                         "read: the streaming-read ceiling kernel")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="--mode spmv: SpMV kernel variant (-1: the library default)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -268,15 +374,20 @@ def main():
     dev = "cuda:0"
     torch.cuda.set_device(0)
 
-    if args.mode == "copy":
+    if args.mode in ("copy", "read"):
         n = 1 << 28  # 2 GiB of doubles
         src = torch.rand(n, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
+        if args.mode == "copy":
+            dst = torch.empty_like(src)
+            fn, written = (lambda: S.copy_f64(dst, src)), 8 * n
+        else:
+            sink = torch.zeros(1, dtype=torch.float64, device=dev)
+            fn, written = (lambda: S.read_f64(src, sink)), 0
         for _ in range(args.warmup):
-            S.copy_f64(dst, src)
-        ms = timed_loop(torch, lambda: S.copy_f64(dst, src), args.steps)
-        print(json.dumps({"mode": "copy", "bytes_read": 8 * n, "bytes_written": 8 * n,
-                          "ms": ms, "GBps": 16 * n / ms / 1e6}))
+            fn()
+        ms = timed_loop(torch, fn, args.steps)
+        print(json.dumps({"mode": args.mode, "bytes_read": 8 * n, "bytes_written": written,
+                          "ms": ms, "GBps": (8 * n + written) / ms / 1e6}))
         return
 
     n = args.n
@@ -301,8 +412,11 @@ def main():
     torch.cuda.synchronize()
 
     if args.mode == "spmv":
+        M.set_kernel(args.variant)
+        M.mult(x, out=y)
         ms = timed_loop(torch, lambda: M.mult(x, out=y), args.steps)
-        print(json.dumps({"mode": "spmv", "ms": ms, "GBps": B / ms / 1e6}))
+        print(json.dumps({"mode": "spmv", "kernel": M.kernel_name(), "ms": ms,
+                          "GBps": B / ms / 1e6}))
         return
 
     # ---- timed steps (the JSON value).  HIP events around each product, on
@@ -330,6 +444,9 @@ def main():
     achieved = B / (spmv_ms * 1e-3) / 1e9
     # the same kernel back to back (no update between), for reference
     spmv_alone_ms = timed_loop(torch, lambda: M.mult(x, out=y), max(args.steps, 20))
+    # the two phases of the reference demo's step (matvecmain.cpp:72-80),
+    # each timed alone: L2GMap::update, then the product
+    update_ms = timed_loop(torch, lambda: l2g.update(x), max(args.steps, 20))
     # PCIe-inclusive step: the halo sent again from the host every step
     # (set_halo: one synchronous host-to-device copy of 204.8 KB)
     npcie = max(args.steps // 2, 10)
@@ -416,6 +533,10 @@ def main():
                      "traffic": traffic, "kernel_ms": round(spmv_ms, 5),
                      "kernel_ms_alone": round(spmv_alone_ms, 5),
                      "kernel": M.kernel_name()},
+        "phases": {"update_ms": round(update_ms, 5), "spmv_ms_in_step": round(spmv_ms, 5),
+                   "step_ms": round(step_ms_ev, 5),
+                   "note": "demos/matvecmain.cpp:72-80 times update and SpMV separately; "
+                           "update alone is timed back to back"},
         "pcie_inclusive": {"ms_per_step": round(pcie_ms, 5), "GBps": round(B / (pcie_ms * 1e-3) / 1e9, 1),
                            "step": "set_halo (204.8 KB host-to-device, synchronous) + update + SpMV"},
         "spmv_sliced": sliced,
@@ -434,7 +555,7 @@ def main():
             reps += 1
         cpu_gbs = B * reps / tc / 1e9
         same = bool(np.array_equal(yc, y.cpu().numpy()))
-        ncores = os.cpu_count() or 1
+        ncores = cpu_threads()
         t = time.perf_counter()
         for _ in range(3):
             O.spmv(oc, xh, threads=ncores)
@@ -460,12 +581,27 @@ def main():
     result["cg_27pt"] = {"seconds": round(tcg, 4), "iterations": k, "rtol": 1e-10,
                          "halo": "zero (homogeneous neighbour values for the direction vector)"}
     log(f"[cg27] {tcg:.3f}s {k} its")
+    if not args.no_cpu:
+        nt = cpu_threads()
+        t = time.perf_counter()
+        xo, ko, _ = O.cg(oc, b.cpu().numpy(), 100000, 1e-10, threads=nt)
+        tco = time.perf_counter() - t
+        result["cg_27pt"]["cpu_all_cores"] = {
+            "threads": nt, "seconds": round(tco, 3), "iterations": ko,
+            "rel_diff_vs_gpu": float(np.linalg.norm(xo - xs.cpu().numpy()) / np.linalg.norm(xo)),
+            "gpu_speedup": round(tco / tcg, 1)}
+        log(f"[cg27] CPU {nt} threads: {tco:.2f}s {ko} its")
+        del xo
     del M, xs
 
+    if not args.no_c1:
+        import oracle as O
+        result["c1_reference_cpu_config"] = run_c1_section(torch, S, P, O, not args.no_cpu,
+                                                           cpu_threads())
     if not args.no_cg:
         import oracle as O
         result["cg"] = run_cg_section(torch, S, P, O, args.cg_n, 5,
-                                      0 if args.no_cpu else 3)
+                                      0 if args.no_cpu else 3, cpu_threads())
     if not args.no_batch:
         result["batch"] = run_batch_section(torch, S, P)
     if not args.no_adjoint:
