@@ -207,6 +207,19 @@ def run_cg_section(torch, S, P, O, n, seed, cpu_iters, nthreads=1):
                      "rel_err_vs_fp64_cg": torch.linalg.norm(x - x64).item() / x64n,
                      "speedup_vs_fp64": t64 / tm}
         log(f"[cg] {name}: {tm:.3f}s outer {st['outer_its']} inner {st['inner_its']}")
+        # variant beside the configuration: each inner solve asks only for
+        # the reduction that brings the outer residual to rtol / 2
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x, st = S.ir_cg(M64, Ml, b, rtol=1e-10, inner_rtol=1e-4, adaptive_inner=True)
+        torch.cuda.synchronize()
+        ta = time.perf_counter() - t
+        out[name + "_adaptive_inner"] = {
+            "seconds": ta, "outer": st["outer_its"], "inner_iterations": st["inner_its"],
+            "inner_per_outer": st["inner_hist"], "true_rel_residual": true_rel(x),
+            "speedup_vs_fp64": t64 / ta,
+            "variant": "inner rtol max(1e-4, 0.5e-10 / outer relative residual)"}
+        log(f"[cg] {name} adaptive: {ta:.3f}s inner {st['inner_its']}")
         Ml.set_kernel(-1)
     if cpu_iters > 0:
         oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
@@ -260,7 +273,8 @@ def run_batch_section(torch, S, P, nsys=256, nrhs=4, rtol=1e-6, seed=17):
     for s_ in list(range(0, nsys, 32)):
         A = systems[s_]
         lo, hi = B.sys_row[s_], B.sys_row[s_ + 1]
-        oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
+        # the system solved: values rounded to FP32
+        oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val.astype(np.float32).astype(np.float64))
         for j in range(nrhs):
             r = bh[lo:hi, j].astype(np.float64) - O.spmv(oc, xh[lo:hi, j].astype(np.float64))
             worst = max(worst, float(np.linalg.norm(r) / np.linalg.norm(bh[lo:hi, j])))
@@ -353,9 +367,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--fd-entries", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--mode", choices=["bench", "spmv", "copy", "read"], default="bench",
+    ap.add_argument("--mode", choices=["bench", "spmv", "copy", "read", "batch"], default="bench",
                     help="spmv: only the SpMV kernel loop; copy: the calibration copy kernel; "
-                         "read: the streaming-read ceiling kernel")
+                         "read: the streaming-read ceiling kernel; batch: the batched solve "
+                         "alone (fixed launch configuration)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="--mode spmv: SpMV kernel variant (-1: the library default)")
     args = ap.parse_args()
@@ -390,6 +405,24 @@ def main():
                           "ms": ms, "GBps": (8 * n + written) / ms / 1e6}))
         return
 
+    if args.mode == "batch":
+        ns = np.exp(np.log(1e3) + (np.log(2e5) - np.log(1e3)) * P.uniform(17, 256)).astype(np.int64)
+        systems = [P.random_spd(int(m), 1000 + i) for i, m in enumerate(ns)]
+        Bt = S.Batch(systems, nrhs=4)
+        Bt.set_config(1024, 4, 4096)
+        b = torch.from_numpy(P.uniform(23, Bt.nrows * 4, -1, 1).astype(np.float32)
+                             .reshape(Bt.nrows, 4)).to(dev)
+        x, its = Bt.solve(b, max_its=20000, rtol=1e-6)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(max(args.steps, 1)):
+            Bt.solve(b, x, 20000, 1e-6, iterations=False)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "batch", "seconds": (time.perf_counter() - t) / max(args.steps, 1),
+                          "its_max": int(its.max()), "its_mean": float(its.mean()),
+                          "config": Bt.config()}))
+        return
+
     n = args.n
     t0 = time.time()
     A = P.stencil(27, n, n, n, ghost_planes=1)
@@ -419,9 +452,12 @@ def main():
                           "GBps": B / ms / 1e6}))
         return
 
-    # ---- timed steps (the JSON value).  HIP events around each product, on
-    # the stream it is launched on (torch's current stream), give the
-    # kernel's average duration inside the timed region (the roofline).
+    # ---- timed steps (the JSON value).  Each product is launched with two
+    # HIP events that the kernel itself records at its start and end
+    # (spmvx_spmv_timed: hipExtLaunchKernel on the stream the product runs
+    # on, torch's current stream), giving the kernel's average duration
+    # inside the timed region (the roofline) without the dispatch gap that
+    # events recorded around the launch would add.
     cur = torch.cuda.current_stream()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for _ in range(args.steps)]
@@ -431,9 +467,7 @@ def main():
     e_all0.record(cur)
     for i in range(args.steps):
         l2g.update(x)
-        ev[i][0].record(cur)
-        M.mult(x, out=y)
-        ev[i][1].record(cur)
+        M.mult_timed(x, y, ev[i][0], ev[i][1], stream=cur)
     e_all1.record(cur)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t
@@ -444,6 +478,35 @@ def main():
     achieved = B / (spmv_ms * 1e-3) / 1e9
     # the same kernel back to back (no update between), for reference
     spmv_alone_ms = timed_loop(torch, lambda: M.mult(x, out=y), max(args.steps, 20))
+
+    def in_step_kernel_ms(nsteps):
+        ev2 = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(nsteps)]
+        torch.cuda.synchronize()
+        for i in range(nsteps):
+            l2g.update(x)
+            M.mult_timed(x, y2, ev2[i][0], ev2[i][1], stream=cur)
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in ev2) / nsteps
+
+    # the same steps with the 32-bit-index CSR kernel (the plain CSR arrays),
+    # when the default reads 16-bit column indices
+    variant, kname = M.kernel()
+    int32_kernel = None
+    if "16-bit" in kname:
+        y2 = torch.empty_like(y)
+        M.set_kernel(int(os.environ.get("SPMV_INT32_VARIANT", "5")))
+        M.mult(x, out=y2)
+        for _ in range(args.warmup):
+            l2g.update(x)
+            M.mult(x, out=y2)
+        i32_ms = in_step_kernel_ms(args.steps)
+        int32_kernel = {"kernel": M.kernel_name(), "kernel_ms": round(i32_ms, 5),
+                        "achieved": round(B / (i32_ms * 1e-3) / 1e9, 1),
+                        "frac": round(B / (i32_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "bit_identical": bool(torch.equal(y, y2))}
+        log(f"[spmv] 32-bit-index kernel in the steps {i32_ms:.4f} ms")
+        M.set_kernel(variant)
     # the two phases of the reference demo's step (matvecmain.cpp:72-80),
     # each timed alone: L2GMap::update, then the product
     update_ms = timed_loop(torch, lambda: l2g.update(x), max(args.steps, 20))
@@ -532,7 +595,14 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel_ms": round(spmv_ms, 5),
                      "kernel_ms_alone": round(spmv_alone_ms, 5),
-                     "kernel": M.kernel_name()},
+                     "kernel": M.kernel_name(),
+                     "hbm_GBps_measured": (round(traffic / (spmv_ms * 1e-3) / 1e9, 1)
+                                           if traffic else None),
+                     "note": ("achieved = the algorithmic CSR bytes (4-byte column indices) "
+                              "per product / kernel time; a kernel reading 16-bit column "
+                              "indices moves fewer bytes (traffic), so hbm_GBps_measured is "
+                              "the bandwidth it actually draws")},
+        "csr_int32_indices": int32_kernel,
         "phases": {"update_ms": round(update_ms, 5), "spmv_ms_in_step": round(spmv_ms, 5),
                    "step_ms": round(step_ms_ev, 5),
                    "note": "demos/matvecmain.cpp:72-80 times update and SpMV separately; "

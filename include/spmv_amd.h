@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define SPMVX_ABI_VERSION 1
+#define SPMVX_ABI_VERSION 2
 
 enum spmvx_status
 {
@@ -174,9 +174,15 @@ int spmvx_matrix_info(spmvx_matrix A, int32_t* dtype, int64_t* nrows, int64_t* n
 int spmvx_matrix_arrays(spmvx_matrix A, const int32_t** rowptr_dev,
                         const int32_t** colidx_dev, const void** values_dev);
 /* Selects the SpMV kernel variant of A (staging chunk, loads in flight,
- * streaming loads; 0 <= variant < spmvx_num_kernels(), -1 = default).  All
- * variants give identical results; used by tuning.  No reference
- * counterpart (the inspector of Matrix.cpp:43 picks its own kernel).
+ * streaming loads, index width; 0 <= variant < spmvx_num_kernels(), -1 =
+ * the matrix's default).  All variants give identical results; used by
+ * tuning.  No reference counterpart (the inspector of Matrix.cpp:43 picks its
+ * own kernel).  Variants 8-11 read 16-bit column indices: for every row block
+ * whose columns fall into at most four windows of 16384 columns (stencils,
+ * banded and locally numbered matrices) the analysis stores each column as a
+ * window number and an offset (2 instead of 4 bytes per entry); other blocks
+ * keep the 32-bit indices.  A matrix's default is the 16-bit counterpart of
+ * its dtype's default kernel when that covers at least half of the entries.
  * The last variant, SPMVX_SLICED_VARIANT, is the sliced row-pattern layout:
  * for matrices whose rows repeat a few column-offset patterns (stencils), a
  * copy of the values is kept in 64-row slices with a 16-bit pattern id per
@@ -184,8 +190,8 @@ int spmvx_matrix_arrays(spmvx_matrix A, const int32_t** rowptr_dev,
  * it builds that copy; a matrix that does not fit the layout (more than
  * 65535 patterns, rows over 256 entries, a pattern table over 8192 entries)
  * returns SPMVX_ERR_ARG with the reason and keeps its variant.  Transposed
- * products use the default CSR variant. */
-#define SPMVX_SLICED_VARIANT 15
+ * products use the default CSR variant when the sliced layout is selected. */
+#define SPMVX_SLICED_VARIANT 12
 int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant);
 int spmvx_num_kernels(void);
 /* How rows longer than a kernel's staging block (2048-8192 entries, per
@@ -217,6 +223,13 @@ int spmvx_matrix_set_values(spmvx_matrix A, const double* values_dev, spmvx_stre
  * must not overlap (SPMVX_ERR_ARG).
  */
 int spmvx_spmv(spmvx_matrix A, const void* x_dev, void* y_dev, spmvx_stream stream);
+
+/* spmvx_spmv, with two hipEvent_t that record the product kernel's own start
+ * and end (hipExtLaunchKernel), so that hipEventElapsedTime between them is
+ * the kernel's execution time without the dispatch before it -- the figure a
+ * kernel trace reports.  A measurement aid; no reference counterpart. */
+int spmvx_spmv_timed(spmvx_matrix A, const void* x_dev, void* y_dev, spmvx_stream stream,
+                     void* start_event, void* stop_event);
 
 /* y[ncols] = A^T x[nrows] -- Matrix<T>::transpmult (Matrix.h:41-43,
  * Matrix.cpp:206-211).  Uses an analysis-time transposed copy; every output
@@ -302,10 +315,14 @@ typedef struct
  * on Alow d = r / ||r|| (Alow: SPMVX_F32 or SPMVX_BF16 copy with the same
  * pattern; FP32 vectors and accumulation; stop at relative residual
  * inner_rtol or inner_max_its) and updates x += ||r|| d in FP64.
- * Every inner solve starts from d = 0, p = r (classic refinement).  flags
- * is reserved and must be 0.
+ * Every inner solve starts from d = 0, p = r (classic refinement).  flags:
+ * 0, or SPMVX_IR_ADAPTIVE_INNER: step j's inner solve stops at relative
+ * residual max(inner_rtol, rtol / (2 ||r_j|| / ||b||)) -- only the reduction
+ * that brings the outer residual to rtol / 2 (the last step usually needs far
+ * fewer inner iterations than inner_rtol asks).
  * x_dev[ncols of A64]; ghost columns are treated as zero.
  */
+#define SPMVX_IR_ADAPTIVE_INNER 1
 int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b_dev, double* x_dev,
                 int32_t max_outer, double rtol, int32_t inner_max_its, double inner_rtol,
                 int32_t flags, spmvx_ir_stats* stats, spmvx_stream stream);
@@ -324,6 +341,9 @@ int spmvx_cg_f32(spmvx_matrix A, const float* b_dev, float* x_dev, int32_t max_i
  * [sys_row[s], sys_row[s+1]); rowptr[sys_row[nsys]+1] and colidx (global
  * column indices, each inside its own system) and FP32 values.  nrhs (1, 2
  * or 4) right-hand sides per system.  Vectors are [row][nrhs] row-major.
+ * With the environment variable SPMVX_BATCH_REORDER=1 at creation, every
+ * system is stored in reverse Cuthill-McKee order (b and x stay in the
+ * caller's order).
  */
 int spmvx_batch_create(int32_t nsys, const int32_t* sys_row_host, const int32_t* rowptr_host,
                        const int32_t* colidx_host, const float* values_host, int32_t nrhs,
@@ -345,8 +365,9 @@ int spmvx_batch_set_config(spmvx_batch B, int32_t block, int32_t lanes_per_row,
  * hipGraph captured on the first call with these arguments.  When the
  * recursive residual meets rtol, the true residual b - A x is formed in
  * double; a right-hand side whose true residual does not meet rtol restarts
- * CG from its x (r = p = b - A x), at most 3 times per solve, so the returned
- * x meets ||b - A x|| <= rtol ||b|| and not only the FP32 recursion.
+ * CG from its x (r = p = b - A x) with a four times lower threshold for the
+ * recursive residual, at most 4 times per solve, so the returned x meets
+ * ||b - A x|| <= rtol ||b|| and not only the FP32 recursion.
  * its_host (may be NULL): nsys*nrhs iteration counts (counted on across
  * restarts).  The stream is synchronised; a timed-out team wait is reported
  * as SPMVX_ERR_STATE (results invalid).

@@ -376,8 +376,8 @@ int or_cg_f64(int64_t nrows, int64_t ncols, const int32_t* rowptr,
  * true residual b - A x is formed with products and sums in double (the FP32
  * values and x are exact in double); if it does not meet rtol either, CG
  * restarts from x with r = p = (float)(b - A x) and ||r||^2 = (float) of its
- * double norm, at most max_restarts times.  Iterations count on across
- * restarts. */
+ * double norm, and a stopping threshold four times lower, at most
+ * max_restarts times.  Iterations count on across restarts. */
 int or_cg_f32_restart(int64_t n, const int32_t* rowptr, const int32_t* colidx,
                       const float* val, const float* b, int kmax, float rtol,
                       float* x, float* hist, int max_restarts)
@@ -401,6 +401,7 @@ int or_cg_f32_restart(int64_t n, const int32_t* rowptr, const int32_t* colidx,
   if (hist)
     hist[0] = rnorm0;
   const float rtol2 = rtol * rtol;
+  float tgt2 = rtol2; /* a quarter of it again after every failed check */
   float rnorm_old = rnorm0;
   int k = 0, restarts = 0;
   while (k < kmax)
@@ -424,7 +425,7 @@ int or_cg_f32_restart(int64_t n, const int32_t* rowptr, const int32_t* colidx,
       p[i] = p[i] * beta + r[i];
     if (hist)
       hist[k] = rnorm_new;
-    if (rnorm_new / rnorm0 < rtol2)
+    if (rnorm_new / rnorm0 < tgt2)
     {
       if (restarts >= max_restarts)
         break;
@@ -443,6 +444,7 @@ int or_cg_f32_restart(int64_t n, const int32_t* rowptr, const int32_t* colidx,
         break;
       ++restarts;
       rnorm_old = (float)rt2;
+      tgt2 *= 0.25f;
     }
   }
   free(r);

@@ -29,7 +29,7 @@ __all__ = ["L2GMap", "Matrix", "Batch", "cg", "cg_f32", "ir_cg", "sddmm", "copy_
 
 F64, F32, BF16, C64, C32 = 0, 1, 2, 3, 4
 # kernel variant of the sliced row-pattern layout (SPMVX_SLICED_VARIANT)
-SLICED_VARIANT = 15
+SLICED_VARIANT = 12
 _DTYPES = {"f64": F64, "float64": F64, "double": F64, F64: F64,
            "f32": F32, "float32": F32, "float": F32, F32: F32,
            "bf16": BF16, "bfloat16": BF16, BF16: BF16,
@@ -103,6 +103,7 @@ def lib():
         "spmvx_matrix_get_kernel": ([P, P, ctypes.c_char_p, i32], ctypes.c_int),
         "spmvx_spmv": ([P, P, P, P], ctypes.c_int),
         "spmvx_spmv_t": ([P, P, P, P], ctypes.c_int),
+        "spmvx_spmv_timed": ([P, P, P, P, P, P], ctypes.c_int),
         "spmvx_cg": ([P, P, P, P, i32, f64, P, P, P], ctypes.c_int),
         "spmvx_cg_f32": ([P, P, P, i32, f64, P, P], ctypes.c_int),
         "spmvx_ir_cg": ([P, P, P, P, i32, f64, i32, f64, i32, P, P], ctypes.c_int),
@@ -366,6 +367,24 @@ class Matrix:
 
     __matmul__ = mult
 
+    def mult_timed(self, x, out, start, stop, stream=None):
+        """mult(x, out) with two torch.cuda.Event(enable_timing=True) that
+        record the product kernel's own start and end: start.elapsed_time(stop)
+        is then the kernel's execution time without its dispatch."""
+        import torch
+        if x.numel() != self.ncols or x.dtype != self.vector_dtype():
+            raise SpmvError("x has the wrong length or dtype")
+        if out.numel() != self.nrows or out.dtype != self.vector_dtype():
+            raise SpmvError("out has the wrong length or dtype")
+        s = stream if stream is not None else torch.cuda.current_stream(x.device)
+        for ev in (start, stop):
+            if not ev.cuda_event:  # torch creates its events at the first record
+                ev.record(s)
+        _check(lib().spmvx_spmv_timed(self._h, _dptr(x), _dptr(out), _stream(s),
+                                      ctypes.c_void_p(start.cuda_event),
+                                      ctypes.c_void_p(stop.cuda_event)))
+        return out
+
     def transpmult(self, x, out=None, stream=None):
         """y = A^T x (Matrix<T>::transpmult, Matrix.cpp:206-211)."""
         import torch
@@ -458,8 +477,10 @@ def cg_f32(A, b, max_its, rtol, stream=None):
 
 
 def ir_cg(A64, Alow, b, rtol=1e-10, inner_rtol=1e-4, max_outer=32, inner_max_its=100000,
-          stream=None):
-    """Mixed-precision iterative refinement: returns (x, stats dict)."""
+          stream=None, adaptive_inner=False):
+    """Mixed-precision iterative refinement: returns (x, stats dict).
+    adaptive_inner: each inner solve stops at max(inner_rtol, rtol / (2 rel))
+    with rel the current outer relative residual (SPMVX_IR_ADAPTIVE_INNER)."""
     import torch
     if b.numel() != A64.nrows or b.dtype != torch.float64:
         raise SpmvError("b must be float64 with A.nrows entries")
@@ -467,7 +488,7 @@ def ir_cg(A64, Alow, b, rtol=1e-10, inner_rtol=1e-4, max_outer=32, inner_max_its
     st = IRStats()
     _check(lib().spmvx_ir_cg(A64.handle, Alow.handle, _dptr(b), _dptr(x), int(max_outer),
                              float(rtol), int(inner_max_its), float(inner_rtol),
-                             0, ctypes.byref(st), _stream(stream)))
+                             1 if adaptive_inner else 0, ctypes.byref(st), _stream(stream)))
     return x, st.as_dict()
 
 

@@ -8,6 +8,7 @@
 // Every kernel returns immediately once the device-side `stopped` flag is
 // set, so the host replays a captured graph of a fixed number of iterations
 // and polls the flag between replays; the iteration count is exact.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -79,7 +80,7 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
            const X* b, X* x, int32_t kmax, Acc rtol, Acc* hist_dev, hipStream_t s)
 {
   Pattern& P = *A->pat;
-  const RowBlocks& rb = variant_blocks(P, A->variant);
+  const CsrView rb = csr_view(P, A->variant);
   ensure_ws<X, Acc>(w, P, P.ncols, rb.nblocks);
   const int64_t n = P.nrows, ncols = P.ncols;
   auto* st = static_cast<CgState<Acc>*>(w.state);
@@ -118,10 +119,9 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
     const int64_t nparts =
         sliced ? launch_sliced<V, X, Acc, kDot>(sv, qq, static_cast<const V*>(vals), p, y,
                                                 nullptr, part_s, st, 1)
-               : launch_csr_stream<V, X, Acc, kDot>(
-                     A->variant, rb.nblocks, qq, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
-                     static_cast<const V*>(vals), p, y, rb.desc.as<int2>(), nullptr, part_s,
-                     st, 1, A->long_rows);
+               : launch_csr_stream<V, X, Acc, kDot>(A->variant, rb, qq,
+                                                    static_cast<const V*>(vals), p, y, nullptr,
+                                                    part_s, st, 1, A->long_rows);
     cg_alpha_finalize<Acc><<<1, kFinBS, 0, qq>>>(part_s, nparts, st, 1);
     cg_update_r<X, Acc><<<vg, kVecBS, 0, qq>>>(n, r, y, part_1, st, 1);
     cg_beta_finalize<Acc><<<1, kFinBS, 0, qq>>>(part_1, vg, st, hist_dev, 1);
@@ -309,11 +309,12 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
     SPMVX_REQUIRE(max_outer >= 0 && max_outer <= 64, "max_outer must be in [0, 64]");
     SPMVX_REQUIRE((b && x) || P.nrows == 0, "NULL vector");
     SPMVX_REQUIRE(inner_max_its >= 0, "inner_max_its must be >= 0");
+    SPMVX_REQUIRE((flags & ~SPMVX_IR_ADAPTIVE_INNER) == 0, "unknown refinement flags");
     hipStream_t s = as_stream(stream);
     const int64_t n = P.nrows;
-    const RowBlocks& rb64 = variant_blocks(P, A64->variant);
+    const CsrView rb64 = csr_view(P, A64->variant);
     ensure_ws<double, double>(A64->ws64, P, P.ncols, rb64.nblocks);
-    const RowBlocks& rb32 = variant_blocks(P, Alow->variant);
+    const CsrView rb32 = csr_view(P, Alow->variant);
     ensure_ws<float, float>(Alow->ws32, P, P.ncols, rb32.nblocks);
     CgWorkspace& w = A64->ws64;
     auto* st = static_cast<CgState<double>*>(w.state);
@@ -325,7 +326,6 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
     spmvx_ir_stats local{};
     auto* flag = static_cast<CgState<double>*>(A64->host_flag.ensure(sizeof(CgState<double>)));
     double bnorm = 0.0;
-    SPMVX_REQUIRE(flags == 0, "no refinement flags are defined");
     int32_t outer = 0;
     for (;;)
     {
@@ -336,8 +336,7 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
                     sliced_view(A64), s, A64->svalues.as<double>(), x, r, b,
                     static_cast<double*>(w.red_spmv), st, 0)
               : launch_csr_stream<double, double, double, kResid>(
-                    A64->variant, rb64.nblocks, s, P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(),
-                    static_cast<const double*>(A64->values.ptr), x, r, rb64.desc.as<int2>(), b,
+                    A64->variant, rb64, s, static_cast<const double*>(A64->values.ptr), x, r, b,
                     static_cast<double*>(w.red_spmv), st, 0, A64->long_rows);
       sum_finalize<double><<<1, kFinBS, 0, s>>>(static_cast<double*>(w.red_spmv), nparts, st);
       check_launch("csr_stream<kResid>");
@@ -352,16 +351,21 @@ int spmvx_ir_cg(spmvx_matrix A64, spmvx_matrix Alow, const double* b, double* x,
       local.rel_residual = rel;
       if (rel <= rtol || outer >= max_outer || rn == 0.0)
         break;
-      // inner FP32 solve on the normalised residual
+      // inner FP32 solve on the normalised residual; with the adaptive flag
+      // an inner solve needs only the reduction that brings the outer
+      // residual to half of rtol (never tighter than inner_rtol)
+      double irtol = inner_rtol;
+      if (flags & SPMVX_IR_ADAPTIVE_INNER)
+        irtol = std::max(inner_rtol, 0.5 * rtol / rel);
       scale_to_f32<<<vg, kVecBS, 0, s>>>(r, r32, n, st);
       check_launch("scale_to_f32");
       int k;
       if (Alow->dtype == SPMVX_F32)
         k = run_cg<float, float, float>(Alow, Alow->ws32, nullptr, SPMVX_F32, r32, d32,
-                                        inner_max_its, static_cast<float>(inner_rtol), nullptr, s);
+                                        inner_max_its, static_cast<float>(irtol), nullptr, s);
       else
         k = run_cg<bf16_t, float, float>(Alow, Alow->ws32, nullptr, SPMVX_F32, r32, d32,
-                                         inner_max_its, static_cast<float>(inner_rtol), nullptr, s);
+                                         inner_max_its, static_cast<float>(irtol), nullptr, s);
       local.inner_hist[outer] = k;
       local.inner_its += k;
       add_correction<<<vg, kVecBS, 0, s>>>(x, d32, n, st);

@@ -75,11 +75,16 @@ struct PinnedBuf
 };
 
 // Row-binning of a pattern for one staging chunk size: block descriptors
-// (first row, first entry), nblocks + 1 of them.
+// (first row, first entry), nblocks + 1 of them; and, built on request for
+// the short-index kernels, every entry's 16-bit column index and every
+// block's four window bases (see csr_stream).
 struct RowBlocks
 {
   DevBuf desc;
   int64_t nblocks = 0;
+  bool short_built = false;
+  int short_kind = 0; // 0: 32-bit only, 1: one base per block, 2: four windows
+  DevBuf col16, wbase;
 };
 
 // CSR pattern + row-binning analysis on the device; shared between a matrix
@@ -153,9 +158,26 @@ inline const RowBlocks* built_variant_blocks(Pattern& P, int var)
   return P.find_blocks(variant_block_nnz(var), variant_block_rows(var));
 }
 
+// Builds the short column indices of a row blocking if missing (device
+// analysis, synchronous).
+void build_short_index(Pattern& P, const RowBlocks& rb, int chunk);
+
+// The arrays a CSR kernel of variant `var` reads (blocks and, for the
+// short-index variants, the short indices are built if missing).
+CsrView csr_view(Pattern& P, int var);
+// The same without building anything: null blocks if they do not exist yet.
+CsrView csr_view_built(Pattern& P, int var);
+
 // Builds the sliced layout of a matrix (row patterns, slices, values) if it
 // is missing; throws with the reason when the pattern is not eligible.
 SlicedView sliced_view(spmvx_matrix_s* A);
+
+inline bool is_complex(int32_t dtype) { return dtype == SPMVX_C64 || dtype == SPMVX_C32; }
+
+// Default kernel variant of a matrix: the short-index counterpart of the
+// dtype's default when the matrix fits 16-bit column indices (builds that
+// analysis), else the dtype's default.
+int matrix_default_variant(spmvx_matrix_s* A);
 
 // Bytes of one stored value of a matrix dtype.
 inline size_t dtype_size(int32_t dtype)

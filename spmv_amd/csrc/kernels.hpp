@@ -20,6 +20,8 @@
 
 #include <initializer_list>
 
+#include <hip/hip_ext.h>
+
 #include "common.hpp"
 
 namespace spmvx
@@ -325,10 +327,20 @@ __device__ __forceinline__ Acc long_row_sum(const int32_t* __restrict__ colidx,
 // blocks hold at most BS rows (the workgroup size) and CHUNK entries.  Block
 // descriptors are
 // (first row, first entry) pairs.
+//
+// Short column indices (IM != 0): when every row block's columns span less
+// than 65536 (IM 1) the analysis stores each column as its offset from the
+// block's first column (wbase.x); otherwise, when they fall into at most four
+// windows of 16384 consecutive columns (IM 2: the planes of a 3D stencil),
+// as (window << 14) | (offset in the window), with the four window bases in
+// wbase.  Stencils, banded and locally numbered matrices fit.  The kernel
+// decodes the same columns in the same order (bit-identical results) while
+// streaming 2 instead of 4 bytes of index per entry.
 template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, int UNROLL, bool NT,
-          int BS>
+          int BS, int IM>
 __global__ __launch_bounds__(BS) void csr_stream(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+    const uint16_t* __restrict__ col16, const int4* __restrict__ wbase, int32_t ncols,
     const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
     const int2* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
     Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it, int32_t long_par)
@@ -352,6 +364,9 @@ __global__ __launch_bounds__(BS) void csr_stream(
     re = rowptr[row + 1];
   }
   Acc part = Acc(0);
+  int4 w = make_int4(0, 0, 0, 0); // base / window bases of a short-index block
+  if constexpr (IM != 0)
+    w = wbase[bi];
 
   if (k1 - k0 <= CHUNK)
   {
@@ -363,7 +378,8 @@ __global__ __launch_bounds__(BS) void csr_stream(
       // the block's first vector, and entries outside the block (neighbouring
       // blocks' entries or the zeroed padding) are valid columns, so every
       // load can be issued before any result is needed; only the LDS store
-      // is predicated.
+      // is predicated.  (A short index decoded with this block's bases is
+      // clamped into [0, ncols): inside the block it already is.)
       int32_t c[UNROLL][VEC];
       V v[UNROLL][VEC];
 #pragma unroll
@@ -371,7 +387,28 @@ __global__ __launch_bounds__(BS) void csr_stream(
       {
         const int32_t g = g0 + u * STEP + tid * VEC;
         const int32_t gs = g < k1 ? g : kb;
-        load_vec<NT>(colidx + gs, c[u]);
+        if constexpr (IM != 0)
+        {
+          uint16_t e[VEC];
+          load_vec<NT>(col16 + gs, e);
+#pragma unroll
+          for (int j = 0; j < VEC; ++j)
+          {
+            const uint32_t q = e[j];
+            int32_t cc;
+            if constexpr (IM == 1)
+              cc = w.x + static_cast<int32_t>(q);
+            else
+            {
+              const uint32_t sel = q >> 14;
+              const int32_t wb = sel == 0 ? w.x : (sel == 1 ? w.y : (sel == 2 ? w.z : w.w));
+              cc = wb + static_cast<int32_t>(q & 0x3FFFu);
+            }
+            c[u][j] = cc < ncols ? cc : ncols - 1;
+          }
+        }
+        else
+          load_vec<NT>(colidx + gs, c[u]);
         load_vec<NT>(val + gs, v[u]);
       }
       X xv[UNROLL][VEC];
@@ -447,463 +484,45 @@ __global__ __launch_bounds__(BS) void csr_stream(
   }
 }
 
-// Software-pipelined variant.  A fixed grid of workgroups (about as many as
-// fit on the chip at once) each walks a contiguous range of row blocks, the
-// ranges of one XCD being adjacent.  While the products of block i are
-// formed (x gathers), summed and stored, the column indices, values and row
-// extents of block i+1 are already in flight in registers, so the matrix
-// stream never pauses.  Blocks hold at most CHUNK - 8 entries here, so that
-// a block's range widened down to a multiple of VEC is covered by PER vector
-// steps.  Per-row arithmetic is the same as csr_stream's (bit-identical).
-template <typename V, int VEC, int PER>
-struct PipeStage
-{
-  int32_t c[PER][VEC];
-  V v[PER][VEC];
-  int32_t ra, re;
-};
-
-template <bool NT, typename V, int VEC, int PER, int BS = kBS>
-__device__ __forceinline__ void pipe_issue(PipeStage<V, VEC, PER>& S,
-                                           const int32_t* __restrict__ rowptr,
-                                           const int32_t* __restrict__ colidx,
-                                           const V* __restrict__ val, int2 d0, int2 d1,
-                                           int long_limit)
-{
-  // Every lane issues the same loads whatever the block (rows past the block
-  // read the first row's extents, a long row's leading entries are loaded
-  // and not used): with no branch around them, the compiler's wait for the
-  // current block's gathers (vmcnt counts loads in issue order) leaves these
-  // loads in flight.  d0.x < d1.x: every block holds a row.
-  (void)long_limit;
-  const int tid = threadIdx.x;
-  const int32_t row = d0.x + tid < d1.x ? d0.x + tid : d0.x;
-  S.ra = rowptr[row];
-  S.re = rowptr[row + 1];
-  const int32_t kb = d0.y & ~(VEC - 1);
-#pragma unroll
-  for (int u = 0; u < PER; ++u)
-  {
-    const int32_t g = kb + (u * BS + tid) * VEC;
-    const int32_t gs = g < d1.y ? g : kb; // stay inside the arrays, no branch
-    load_vec<NT>(colidx + gs, S.c[u]);
-    load_vec<NT>(val + gs, S.v[u]);
-  }
-}
-
-// x gathers of a staged block.  Issued before the next block's loads so that
-// waiting for them (vmcnt counts loads in issue order) does not also wait for
-// the prefetch.
-template <typename V, typename X, int VEC, int PER>
-__device__ __forceinline__ void pipe_gather(const PipeStage<V, VEC, PER>& S,
-                                            const X* __restrict__ x, int2 d0, int2 d1,
-                                            X (&xg)[PER][VEC], int long_limit)
-{
-  (void)d0;
-  (void)d1;
-  (void)long_limit;
-#pragma unroll
-  for (int u = 0; u < PER; ++u)
-#pragma unroll
-    for (int j = 0; j < VEC; ++j)
-      xg[u][j] = x[S.c[u][j]]; // every loaded column is valid (see pipe_issue)
-}
-
-template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, int PER, bool NT>
-__device__ __forceinline__ void pipe_compute(const PipeStage<V, VEC, PER>& S,
-                                             const X (&xg)[PER][VEC],
-                                             const int32_t* __restrict__ colidx,
-                                             const V* __restrict__ val,
-                                             const X* __restrict__ x, X* __restrict__ y,
-                                             const X* __restrict__ b, int2 d0, int2 d1,
-                                             Acc* prod, Acc* red, Acc& part, int long_limit,
-                                             int32_t long_par)
-{
-  const int tid = threadIdx.x;
-  const int32_t r0 = d0.x, r1 = d1.x, k0 = d0.y, k1 = d1.y;
-  const int32_t row = r0 + tid;
-  if (k1 - k0 <= long_limit)
-  {
-    const int32_t kb = k0 & ~(VEC - 1);
-#pragma unroll
-    for (int u = 0; u < PER; ++u)
-    {
-      const int32_t g = kb + (u * kBS + tid) * VEC;
-#pragma unroll
-      for (int j = 0; j < VEC; ++j)
-      {
-        const int32_t k = g + j; // outside the block: spare slot prod[CHUNK]
-        const bool in = g < k1 && k >= k0 && k < k1;
-        prod[in ? k - k0 : CHUNK] =
-            static_cast<Acc>(widen(S.v[u][j])) * static_cast<Acc>(xg[u][j]);
-      }
-    }
-    lds_barrier();
-    if (row < r1)
-    {
-      const Acc s = row_sum(prod, S.ra - k0, S.re - k0);
-      if (MODE == kResid)
-      {
-        const X yi = b[row] - static_cast<X>(s);
-        store_result(y + row, yi);
-        part += static_cast<Acc>(yi) * static_cast<Acc>(yi);
-      }
-      else
-      {
-        const X yi = static_cast<X>(s);
-        store_result(y + row, yi);
-        if (MODE == kDot)
-          part += static_cast<Acc>(x[row]) * static_cast<Acc>(yi);
-      }
-    }
-    lds_barrier();
-  }
-  else
-  {
-    // one long row
-    const Acc s =
-        long_row_sum<V, X, Acc, CHUNK, kBS, NT>(colidx, val, x, k0, k1, prod, red, long_par);
-    if (tid == 0)
-    {
-      if (MODE == kResid)
-      {
-        const X yi = b[r0] - static_cast<X>(s);
-        store_result(y + r0, yi);
-        part += static_cast<Acc>(yi) * static_cast<Acc>(yi);
-      }
-      else
-      {
-        const X yi = static_cast<X>(s);
-        store_result(y + r0, yi);
-        if (MODE == kDot)
-          part += static_cast<Acc>(x[r0]) * static_cast<Acc>(yi);
-      }
-    }
-  }
-}
-
-template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, bool NT>
-__global__ __launch_bounds__(kBS) void csr_pipe(
-    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
-    const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
-    const int2* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
-    Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it, int32_t long_par)
-{
-  if (MODE != kPlain && !cg_live(st, it))
-    return;
-  constexpr int PER = CHUNK / (kBS * VEC);
-  static_assert(PER * kBS * VEC == CHUNK, "CHUNK must be a multiple of kBS*VEC");
-  constexpr int kLong = CHUNK - 8;
-  __shared__ Acc prod[CHUNK + 1];
-  __shared__ Acc red[16];
-  const int64_t w = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t bb = nblocks * w / gridDim.x, be = nblocks * (w + 1) / gridDim.x;
-  Acc part = Acc(0);
-  PipeStage<V, VEC, PER> A, B;
-  X xg[PER][VEC];
-  if (bb < be)
-  {
-    // block bi is [a0, a1); a2 ends block bi+1 (loaded one iteration ahead);
-    // the next block's loads are unconditional (the current block again
-    // after the last one), see pipe_issue
-    int2 a0 = blocks[bb], a1 = blocks[bb + 1];
-    int2 a2 = blocks[bb + 2 < be ? bb + 2 : be];
-    pipe_issue<NT>(A, rowptr, colidx, val, a0, a1, kLong);
-    int64_t bi = bb;
-    auto step = [&](PipeStage<V, VEC, PER>& cur, PipeStage<V, VEC, PER>& nxt) {
-      pipe_gather(cur, x, a0, a1, xg, kLong);
-      const bool more = bi + 1 < be;
-      pipe_issue<NT>(nxt, rowptr, colidx, val, more ? a1 : a0, more ? a2 : a1, kLong);
-      const int2 a3 = blocks[bi + 3 < be ? bi + 3 : be];
-      pipe_compute<V, X, Acc, MODE, CHUNK, VEC, PER, NT>(cur, xg, colidx, val, x, y, b, a0, a1,
-                                                          prod, red, part, kLong, long_par);
-      a0 = a1;
-      a1 = a2;
-      a2 = a3;
-      return ++bi < be;
-    };
-    while (step(A, B) && step(B, A))
-    {
-    }
-  }
-  if (MODE == kPlain)
-    return;
-  part = block_sum<Acc, true>(part, red);
-  if (threadIdx.x == 0)
-    partials[w] = part;
-}
-
-// Persistent variant that holds its results in LDS.  OCC workgroups per CU
-// (sharing its LDS) each walk a sequence of row blocks with the next block's
-// column indices, values and row extents in flight in registers, as
-// csr_pipe does; the row sums go to an LDS buffer that is written to y only
-// when it is full, and at the end.  On a 27-point share (8,000 rows per
-// workgroup, 7,872 held) the results are written in two bursts near the
-// end: a stream of
-// reads interleaved with a 2.4% trickle of writes ran ~20% slower than the
-// same reads with the writes held back and written last (probe kernels on
-// MI355X).  The workgroups of one XCD take that XCD's contiguous range of
-// blocks in turn (slot j: blocks j, j + slots, ...), so they move through
-// memory together (2.4% faster than a contiguous range per workgroup).
-// Per-row arithmetic is csr_stream's (bit-identical).
-// Measured on the 27-point share: 0.246 ms alone (csr_stream 0.255 ms), but
-// no faster inside update + product and 10-20% slower in CG, where the
-// static split cannot absorb the skew that the dynamic dispatch of
-// csr_stream does; it is a tuning option, not a default.
-constexpr int kLdsBytes = 160 * 1024; // LDS per CU on gfx950
-constexpr int kDeferRuns = 256;        // row runs the result buffer tracks
-
-// Rows the result buffer of csr_defer holds (a multiple of 64) when OCC
-// workgroups share a CU's LDS.
-template <typename X, typename Acc, int CHUNK, int OCC>
-constexpr int defer_ycap()
-{
-  return ((kLdsBytes / OCC - (CHUNK + 1 + 16) * static_cast<int>(sizeof(Acc))
-           - (kDeferRuns + 1) * 8 - 256)
-          / static_cast<int>(sizeof(X)))
-         & ~63;
-}
-
-// Result buffer of csr_defer: `fill` slots of ybuf hold runs of consecutive
-// rows; run k starts at row run[k].x and slot run[k].y.  Every thread keeps
-// the same copy of the counters.
-template <typename X>
-struct DeferBuf
-{
-  X* ybuf;
-  int2* run;
-  int32_t nrun, fill, last; // last = end row of the last run
-};
-
-// Writes the buffered rows to y and empties the buffer.  Called by all
-// threads after a barrier; ends with one (the buffer is reused next).
-template <typename X, int BS>
-__device__ __forceinline__ void defer_flush(DeferBuf<X>& D, X* __restrict__ y)
-{
-  const int tid = threadIdx.x;
-  constexpr int NW = BS / 64;
-  if (D.nrun >= NW)
-  {
-    // many short runs: one wavefront per run
-    for (int k = tid / 64; k < D.nrun; k += NW)
-    {
-      const int2 r = D.run[k];
-      const int32_t e = k + 1 < D.nrun ? D.run[k + 1].y : D.fill;
-      for (int32_t i = r.y + (tid & 63); i < e; i += 64)
-        y[r.x + (i - r.y)] = D.ybuf[i];
-    }
-  }
-  else
-  {
-    for (int k = 0; k < D.nrun; ++k)
-    {
-      const int2 r = D.run[k];
-      const int32_t e = k + 1 < D.nrun ? D.run[k + 1].y : D.fill;
-      for (int32_t i = r.y + tid; i < e; i += BS)
-        y[r.x + (i - r.y)] = D.ybuf[i];
-    }
-  }
-  D.nrun = 0;
-  D.fill = 0;
-  D.last = -1;
-  lds_barrier();
-}
-
-template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, int PER, int OCC,
-          int BS, bool NT>
-__device__ __forceinline__ void defer_compute(const PipeStage<V, VEC, PER>& S,
-                                              const X (&xg)[PER][VEC],
-                                              const int32_t* __restrict__ colidx,
-                                              const V* __restrict__ val,
-                                              const X* __restrict__ x, X* __restrict__ y,
-                                              X aux, int2 d0, int2 d1, Acc* prod, Acc* red,
-                                              DeferBuf<X>& D, Acc& part, int32_t long_par)
-{
-  // aux: x[row] (kDot) or b[row] (kResid) of this thread's row, loaded by the
-  // caller before the gathers (a load issued here would wait for the next
-  // block's loads, which retire first)
-  constexpr int YCAP = defer_ycap<X, Acc, CHUNK, OCC>();
-  constexpr int kLong = CHUNK - 8;
-  const int tid = threadIdx.x;
-  const int32_t r0 = d0.x, r1 = d1.x, k0 = d0.y, k1 = d1.y;
-  // rows held so far are complete (the previous block ended with a barrier)
-  if (D.fill + (r1 - r0) > YCAP || (D.nrun == kDeferRuns && r0 != D.last))
-    defer_flush<X, BS>(D, y);
-  if (r0 != D.last)
-  {
-    if (tid == 0)
-      D.run[D.nrun] = make_int2(r0, D.fill);
-    ++D.nrun;
-  }
-  X* yb = D.ybuf + D.fill; // slot of row r: yb[r - r0]
-  D.fill += r1 - r0;
-  D.last = r1;
-  const int32_t row = r0 + tid;
-  if (k1 - k0 <= kLong)
-  {
-    const int32_t kb = k0 & ~(VEC - 1);
-#pragma unroll
-    for (int u = 0; u < PER; ++u)
-    {
-      const int32_t g = kb + (u * BS + tid) * VEC;
-#pragma unroll
-      for (int j = 0; j < VEC; ++j)
-      {
-        const int32_t k = g + j; // outside the block: spare slot prod[CHUNK]
-        const bool in = g < k1 && k >= k0 && k < k1;
-        prod[in ? k - k0 : CHUNK] =
-            static_cast<Acc>(widen(S.v[u][j])) * static_cast<Acc>(xg[u][j]);
-      }
-    }
-    lds_barrier();
-    if (row < r1)
-    {
-      const Acc s = row_sum(prod, S.ra - k0, S.re - k0);
-      const X yi = MODE == kResid ? aux - static_cast<X>(s) : static_cast<X>(s);
-      yb[row - r0] = yi;
-      if (MODE == kResid)
-        part += static_cast<Acc>(yi) * static_cast<Acc>(yi);
-      if (MODE == kDot)
-        part += static_cast<Acc>(aux) * static_cast<Acc>(yi);
-    }
-    lds_barrier();
-  }
-  else
-  {
-    // one long row
-    const Acc s =
-        long_row_sum<V, X, Acc, CHUNK, BS, NT>(colidx, val, x, k0, k1, prod, red, long_par);
-    if (tid == 0) // row r0 is thread 0's row
-    {
-      const X yi = MODE == kResid ? aux - static_cast<X>(s) : static_cast<X>(s);
-      yb[0] = yi;
-      if (MODE == kResid)
-        part += static_cast<Acc>(yi) * static_cast<Acc>(yi);
-      if (MODE == kDot)
-        part += static_cast<Acc>(aux) * static_cast<Acc>(yi);
-    }
-    lds_barrier(); // the result is in the buffer before any thread flushes it
-  }
-}
-
-template <typename V, typename X, typename Acc, int MODE, int CHUNK, int VEC, bool NT, int BS,
-          int OCC>
-__global__ __launch_bounds__(BS, OCC* BS / 256) void csr_defer( // OCC workgroups' waves per SIMD
-    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
-    const V* __restrict__ val, const X* __restrict__ x, X* __restrict__ y,
-    const int2* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
-    Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it, int32_t long_par)
-{
-  if (MODE != kPlain && !cg_live(st, it))
-    return;
-  constexpr int PER = CHUNK / (BS * VEC);
-  static_assert(PER * BS * VEC == CHUNK, "CHUNK must be a multiple of BS*VEC");
-  constexpr int kLong = CHUNK - 8;
-  __shared__ Acc prod[CHUNK + 1];
-  __shared__ X ybuf[defer_ycap<X, Acc, CHUNK, OCC>()];
-  __shared__ int2 run[kDeferRuns];
-  __shared__ Acc red[16];
-  const int tid = threadIdx.x;
-  // block sequence first, first + stride, ... < end: this XCD's share of the
-  // blocks (in proportion to its workgroups), taken in turn by its workgroups
-  const int64_t w = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t xcd = blockIdx.x % kNumXcd, slot = blockIdx.x / kNumXcd;
-  const int64_t q = gridDim.x / kNumXcd, rem = gridDim.x % kNumXcd;
-  const int64_t g0 = xcd * q + (xcd < rem ? xcd : rem); // first workgroup of this XCD
-  const int64_t stride = q + (xcd < rem ? 1 : 0);
-  const int64_t first = nblocks * g0 / gridDim.x + slot;
-  const int64_t end = nblocks * (g0 + stride) / gridDim.x;
-  Acc part = Acc(0);
-  DeferBuf<X> D{ybuf, run, 0, 0, -1};
-  if (first < end)
-  {
-    // Two register stages: the next block's loads are issued after the
-    // current block's gathers and stay in flight while the gathers return
-    // and the block is summed.  (More stages would not help: loads retire in
-    // issue order, so waiting for the gathers waits for every load issued
-    // before them.)  A step past the sequence reloads the current block, so
-    // every step issues the same loads.  Descriptors: the current block
-    // [c0, c1) and the next [n0, n1), loaded one step ahead.
-    PipeStage<V, VEC, PER> A, B;
-    X xg[PER][VEC];
-    int64_t bi = first;
-    int2 c0 = blocks[bi], c1 = blocks[bi + 1];
-    const int64_t nb = bi + stride < end ? bi + stride : bi;
-    int2 n0 = blocks[nb], n1 = blocks[nb + 1];
-    pipe_issue<NT, V, VEC, PER, BS>(A, rowptr, colidx, val, c0, c1, kLong);
-    auto step = [&](PipeStage<V, VEC, PER>& cur, PipeStage<V, VEC, PER>& nxt) {
-      X aux = X(0);
-      if (MODE != kPlain && c0.x + tid < c1.x)
-        aux = MODE == kDot ? x[c0.x + tid] : b[c0.x + tid];
-      pipe_gather(cur, x, c0, c1, xg, kLong);
-      const bool more = bi + stride < end;
-      pipe_issue<NT, V, VEC, PER, BS>(nxt, rowptr, colidx, val, more ? n0 : c0,
-                                      more ? n1 : c1, kLong);
-      const int64_t ab = bi + 2 * stride < end ? bi + 2 * stride : bi;
-      const int2 a0 = blocks[ab], a1 = blocks[ab + 1];
-      defer_compute<V, X, Acc, MODE, CHUNK, VEC, PER, OCC, BS, NT>(
-          cur, xg, colidx, val, x, y, aux, c0, c1, prod, red, D, part, long_par);
-      c0 = n0;
-      c1 = n1;
-      n0 = a0;
-      n1 = a1;
-      bi += stride;
-      return bi < end;
-    };
-    while (step(A, B) && step(B, A))
-    {
-    }
-    defer_flush<X, BS>(D, y);
-  }
-  if (MODE == kPlain)
-    return;
-  part = block_sum<Acc, true>(part, red);
-  if (threadIdx.x == 0)
-    partials[w] = part;
-}
-
-// Kernel variants selectable per matrix: {kind (0 one block per workgroup,
-// 1 pipelined, 2 pipelined with results held in LDS), chunk, vec, unroll,
-// non-temporal, threads per workgroup}.
+// Kernel variants selectable per matrix: {chunk (entries staged per block),
+// vec (consecutive entries per vector load), unroll (vector steps in flight),
+// non-temporal matrix loads, threads per workgroup (= rows per block at
+// most), short column indices}.  Variants 0-4 also take complex values.
 struct SpmvVariant
 {
-  int kind;
   int chunk;
   int vec;
-  int unroll; // kind 2: unused (1)
+  int unroll;
   bool nt;
-  int bs;  // threads per workgroup = rows per block at most
-  int occ; // kind 2: workgroups per CU (LDS shared between them)
+  int bs;
+  bool i16;
 };
 #define SPMVX_VARIANTS(X)                                                                    \
-  X(0, 0, 2048, 1, 1, false, 256, 0)                                                         \
-  X(1, 0, 2048, 1, 4, true, 256, 0)                                                          \
-  X(2, 0, 2048, 2, 2, true, 256, 0)                                                          \
-  X(3, 0, 2048, 4, 1, true, 256, 0)                                                          \
-  X(4, 0, 2048, 4, 2, true, 256, 0)                                                          \
-  X(5, 1, 2048, 2, 1, true, 256, 0)                                                          \
-  X(6, 1, 2048, 4, 1, true, 256, 0)                                                          \
-  X(7, 1, 2048, 8, 1, true, 256, 0)                                                          \
-  X(8, 1, 4096, 4, 1, true, 256, 0)                                                          \
-  X(9, 1, 1024, 4, 1, true, 256, 0)                                                          \
-  X(10, 1, 2048, 4, 1, false, 256, 0)                                                        \
-  X(11, 0, 8192, 2, 4, true, 1024, 0)                                                        \
-  X(12, 0, 8192, 2, 2, true, 1024, 0)                                                        \
-  X(13, 0, 8192, 4, 2, true, 1024, 0)                                                        \
-  X(14, 2, 2048, 2, 1, true, 1024, 2)
-constexpr int kNumVariants = 15;
-#define SPMVX_VARIANT_ENTRY(i, K, C, V, U, N, B, O) {K, C, V, U, N, B, O},
+  X(0, 2048, 1, 1, false, 256, false)                                                        \
+  X(1, 2048, 1, 4, true, 256, false)                                                         \
+  X(2, 2048, 2, 2, true, 256, false)                                                         \
+  X(3, 2048, 4, 1, true, 256, false)                                                         \
+  X(4, 2048, 4, 2, true, 256, false)                                                         \
+  X(5, 8192, 2, 4, true, 1024, false)                                                        \
+  X(6, 8192, 2, 2, true, 1024, false)                                                        \
+  X(7, 8192, 4, 2, true, 1024, false)                                                        \
+  X(8, 8192, 2, 4, true, 1024, true)                                                         \
+  X(9, 2048, 4, 1, true, 256, true)                                                          \
+  X(10, 2048, 4, 2, true, 256, true)                                                         \
+  X(11, 2048, 2, 4, true, 256, true)
+constexpr int kNumVariants = 12;
+#define SPMVX_VARIANT_ENTRY(i, C, V, U, N, B, I) {C, V, U, N, B, I},
 constexpr SpmvVariant kVariants[kNumVariants] = {SPMVX_VARIANTS(SPMVX_VARIANT_ENTRY)};
 #undef SPMVX_VARIANT_ENTRY
 
-// Entries per row block of a variant (pipelined kernels keep 8 spare slots).
-// The sliced layout (kSlicedVariant, defined below) reports the blocks of
-// variant 0, which CSR-only operations (transpose, SDDMM) then use.
+// Entries and rows per row block of a variant.  The sliced layout
+// (kSlicedVariant, defined below) reports the blocks of variant 0, which
+// CSR-only operations (transpose, SDDMM) then use.
 inline int variant_block_nnz(int var)
 {
   if (var < 0 || var >= kNumVariants)
     var = 0;
-  return kVariants[var].kind != 0 ? kVariants[var].chunk - 8 : kVariants[var].chunk;
+  return kVariants[var].chunk;
 }
 inline int variant_block_rows(int var)
 {
@@ -911,55 +530,80 @@ inline int variant_block_rows(int var)
     var = 0;
   return kVariants[var].bs;
 }
+inline bool variant_short(int var) { return var >= 0 && var < kNumVariants && kVariants[var].i16; }
 
-// Launches the SpMV kernel of variant `var` over `nblocks` row blocks (built
-// with variant_block_nnz(var) entries per block); returns the number of
-// partial sums written (MODE != kPlain).  long_par: rows longer than a block
-// are summed in parallel (long_row_sum) instead of in stored order.
-int pipe_grid(const void* kernel, int64_t nblocks, int threads = kBS);
-
-template <typename V, typename X, typename Acc, int MODE>
-int64_t launch_csr_stream(int var, int64_t nblocks, hipStream_t s, const int32_t* rowptr,
-                          const int32_t* colidx, const V* val, const X* x, X* y,
-                          const int2* blocks, const X* b, Acc* partials,
-                          const CgState<Acc>* st, int32_t it, int32_t long_par = 0)
+// Device arrays of a CSR matrix for one row blocking: the CSR arrays, the
+// block descriptors and, for the short-index variants, the 16-bit column
+// indices and the per-block window bases (null when not built).
+struct CsrView
 {
-  if (nblocks <= 0)
+  const int32_t* rowptr;
+  const int32_t* colidx;
+  const int2* blocks;
+  int64_t nblocks;
+  const uint16_t* col16;
+  const int4* wbase;
+  int32_t ncols;
+  int short_kind; // 1: one base per block, 2: four windows per block
+};
+
+// Launches kernel with args on stream s; when ev0 / ev1 are given they
+// record the kernel's own start and end (hipExtLaunchKernel), so that their
+// elapsed time is the kernel's execution without the dispatch before it.
+template <typename... KArgs, typename... Args>
+inline void launch_timed(void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32_t shmem,
+                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, Args... args)
+{
+  if (ev0 || ev1)
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, ev0, ev1, 0, static_cast<KArgs>(args)...);
+  else
+    kernel<<<grid, block, shmem, s>>>(static_cast<KArgs>(args)...);
+}
+
+// Launches the SpMV kernel of variant `var` (blocks built with
+// variant_block_nnz/rows(var)); returns the number of partial sums written
+// (MODE != kPlain).  long_par: rows longer than a block are summed in
+// parallel (long_row_sum) instead of in stored order.  ev0 / ev1: see
+// launch_timed.
+template <typename V, typename X, typename Acc, int MODE>
+int64_t launch_csr_stream(int var, const CsrView& M, hipStream_t s, const V* val, const X* x,
+                          X* y, const X* b, Acc* partials, const CgState<Acc>* st, int32_t it,
+                          int32_t long_par = 0, hipEvent_t ev0 = nullptr,
+                          hipEvent_t ev1 = nullptr)
+{
+  if (M.nblocks <= 0)
     return 0;
-  int64_t grid = nblocks;
-#define SPMVX_CASE(i, K, C, VV, U, N, B, O)                                                  \
+  const dim3 grid(static_cast<unsigned>(M.nblocks));
+#define SPMVX_LAUNCH_IM(IMV)                                                                 \
+  launch_timed(&csr_stream<V, X, Acc, MODE, C, VV, U, N, B, IMV>, grid, dim3(B), 0, s, ev0,   \
+               ev1, M.rowptr, M.colidx, M.col16, M.wbase, M.ncols, val, x, y, M.blocks,       \
+               M.nblocks, b, partials, st, it, long_par)
+#define SPMVX_CASE(i, CC, VVV, UU, NN, BB, I)                                                \
   case i:                                                                                    \
-    if constexpr (kIsComplex<V> && (K != 0 || C > 2048))                                     \
+  {                                                                                          \
+    constexpr int C = CC, VV = VVV, U = UU, B = BB;                                          \
+    constexpr bool N = NN;                                                                   \
+    if constexpr (kIsComplex<V> && (C > 2048 || I))                                          \
       throw Error(SPMVX_ERR_ARG, "kernel variant not available for complex values");         \
-    else if constexpr (K == 0)                                                               \
-      csr_stream<V, X, Acc, MODE, C, VV, U, N, B><<<dim3(static_cast<unsigned>(grid)),       \
-                                                    dim3(B), 0, s>>>(                       \
-          rowptr, colidx, val, x, y, blocks, nblocks, b, partials, st, it, long_par);        \
-    else if constexpr (K == 2)                                                               \
+    else if constexpr (I)                                                                    \
     {                                                                                        \
-      grid = pipe_grid(                                                                      \
-          reinterpret_cast<const void*>(&csr_defer<V, X, Acc, MODE, C, VV, N, B, O>),           \
-          nblocks, B);                                                                       \
-      csr_defer<V, X, Acc, MODE, C, VV, N, B, O><<<dim3(static_cast<unsigned>(grid)),        \
-                                                      dim3(B), 0, s>>>(                      \
-          rowptr, colidx, val, x, y, blocks, nblocks, b, partials, st, it, long_par);        \
+      if (M.short_kind == 1)                                                                 \
+        SPMVX_LAUNCH_IM(1);                                                                  \
+      else                                                                                   \
+        SPMVX_LAUNCH_IM(2);                                                                  \
     }                                                                                        \
     else                                                                                     \
-    {                                                                                        \
-      grid = pipe_grid(reinterpret_cast<const void*>(&csr_pipe<V, X, Acc, MODE, C, VV, N>),  \
-                       nblocks);                                                             \
-      csr_pipe<V, X, Acc, MODE, C, VV, N><<<dim3(static_cast<unsigned>(grid)), dim3(kBS), 0, \
-                                            s>>>(rowptr, colidx, val, x, y, blocks, nblocks, \
-                                                 b, partials, st, it, long_par);             \
-    }                                                                                        \
-    break;
+      SPMVX_LAUNCH_IM(0);                                                                    \
+    break;                                                                                   \
+  }
   switch (var)
   {
     SPMVX_VARIANTS(SPMVX_CASE)
   default: break;
   }
 #undef SPMVX_CASE
-  return grid;
+#undef SPMVX_LAUNCH_IM
+  return M.nblocks;
 }
 
 // ---------------------------------------------------------------- sliced row patterns
@@ -1136,11 +780,36 @@ inline int sliced_step(int w, std::initializer_list<int> sizes)
   return best;
 }
 
+// Value loads per step (UNR) of the sliced kernel for values of vbytes bytes,
+// rpl slices per wavefront and the padded pattern width wpad (slots): at
+// each value type's default rpl, the compiled step (UNR x VW slots) that
+// leaves the fewest idle slots, then the fewest steps (the best of the sizes
+// compiled on each of the 27- and 7-point stencils and banded matrices of
+// widths 5-41 measured); with rpl set by hand, the fixed step.
+inline int sliced_unr(int vbytes, int rpl, int wpad)
+{
+  if (vbytes == 8)
+  {
+    if (rpl == 1)
+      return sliced_step(wpad, {4, 8, 10, 14});
+    return 4;
+  }
+  if (vbytes == 4)
+  {
+    if (rpl == 4)
+      return sliced_step(wpad, {2, 4}) / 2;
+    return 2;
+  }
+  if (rpl == 2)
+    return sliced_step(wpad, {4, 8, 12}) / 4;
+  return 2;
+}
+
 template <typename V, typename X, typename Acc, int MODE>
 int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X* x, X* y,
-                      const X* b, Acc* partials, const CgState<Acc>* st, int32_t it)
+                      const X* b, Acc* partials, const CgState<Acc>* st, int32_t it,
+                      hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
-  constexpr int UNR = sizeof(V) == 8 ? 4 : 2;
   const size_t lds = sizeof(int32_t) * static_cast<size_t>(S.dict_n);
   const int spw = (kSliceBS / kSlice) * S.rpl;
   const int64_t nblk = (S.nslices + spw - 1) / spw;
@@ -1148,23 +817,19 @@ int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X*
     return 0;
   const dim3 grid(static_cast<unsigned>(nblk));
   auto go = [&](auto unr, auto rpl) {
-    sliced_spmv<V, X, Acc, MODE, decltype(unr)::value, decltype(rpl)::value>
-        <<<grid, dim3(kSliceBS), lds, s>>>(S.pid, S.sdesc, S.nslices, S.nrows, S.ncols, S.dict,
-                                           S.dict_n, S.wmax, val, x, y, b, partials, st, it);
+    launch_timed(&sliced_spmv<V, X, Acc, MODE, decltype(unr)::value, decltype(rpl)::value>, grid,
+                 dim3(kSliceBS), static_cast<uint32_t>(lds), s, ev0, ev1, S.pid, S.sdesc,
+                 S.nslices, S.nrows, S.ncols, S.dict, S.dict_n, S.wmax, val, x, y, b, partials, st,
+                 it);
   };
   using std::integral_constant;
   constexpr int VW = sliced_vw<V>();
   const int wpad = (S.wmax + VW - 1) / VW * VW; // slice width in slots
-  // Slots loaded per step at each value type's default slices per wavefront,
-  // chosen from the pattern width: the step size that leaves the fewest idle
-  // slots in the padded width, then the fewest steps (the best of the sizes
-  // compiled on each of the 27- and 7-point stencils and banded matrices of
-  // widths 5-41 measured).
+  const int u = sliced_unr(static_cast<int>(sizeof(V)), S.rpl, wpad);
   if constexpr (sizeof(V) == 8)
   {
     if (S.rpl == 1)
     {
-      const int u = sliced_step(wpad, {4, 8, 10, 14});
       if (u == 14)
         go(integral_constant<int, 14>{}, integral_constant<int, 1>{});
       else if (u == 10)
@@ -1173,41 +838,42 @@ int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X*
         go(integral_constant<int, 8>{}, integral_constant<int, 1>{});
       else
         go(integral_constant<int, 4>{}, integral_constant<int, 1>{});
-      return nblk;
     }
+    else if (S.rpl == 2)
+      go(integral_constant<int, 4>{}, integral_constant<int, 2>{});
+    else
+      go(integral_constant<int, 4>{}, integral_constant<int, 4>{});
   }
   else if constexpr (sizeof(V) == 4)
   {
     if (S.rpl == 4)
     {
-      if (sliced_step(wpad, {2, 4}) == 4)
+      if (u == 2)
         go(integral_constant<int, 2>{}, integral_constant<int, 4>{});
       else
         go(integral_constant<int, 1>{}, integral_constant<int, 4>{});
-      return nblk;
     }
+    else if (S.rpl == 2)
+      go(integral_constant<int, 2>{}, integral_constant<int, 2>{});
+    else
+      go(integral_constant<int, 2>{}, integral_constant<int, 1>{});
   }
   else
   {
     if (S.rpl == 2)
     {
-      const int u = sliced_step(wpad, {4, 8, 12});
-      if (u == 12)
+      if (u == 3)
         go(integral_constant<int, 3>{}, integral_constant<int, 2>{});
-      else if (u == 8)
+      else if (u == 2)
         go(integral_constant<int, 2>{}, integral_constant<int, 2>{});
       else
         go(integral_constant<int, 1>{}, integral_constant<int, 2>{});
-      return nblk;
     }
+    else if (S.rpl == 4)
+      go(integral_constant<int, 2>{}, integral_constant<int, 4>{});
+    else
+      go(integral_constant<int, 2>{}, integral_constant<int, 1>{});
   }
-  // slices per wavefront set by hand (SPMVX_SLICED_RPL): the fixed step
-  if (S.rpl == 4)
-    go(integral_constant<int, UNR>{}, integral_constant<int, 4>{});
-  else if (S.rpl == 2)
-    go(integral_constant<int, UNR>{}, integral_constant<int, 2>{});
-  else
-    go(integral_constant<int, UNR>{}, integral_constant<int, 1>{});
   return nblk;
 }
 

@@ -178,37 +178,194 @@ const SliceLayout& Pattern::slice_layout(int vw)
   return ref;
 }
 
-int pipe_grid(const void* kernel, int64_t nblocks, int threads)
+namespace
 {
-  // all workgroups resident at once: CUs x the kernel's occupancy
-  static int ncu = [] {
-    int dev = 0, n = 0;
-    if (hipGetDevice(&dev) != hipSuccess
-        || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      n = 256;
-    return n > 0 ? n : 256;
-  }();
-  static std::map<const void*, int> occ;
-  static std::mutex occ_mu;
-  std::lock_guard<std::mutex> lock(occ_mu);
-  auto it = occ.find(kernel);
-  if (it == occ.end())
+
+// Short-index analysis, one thread per row block of at most `chunk` entries
+// (longer blocks hold one row each, which the kernels sum from the 32-bit
+// indices): the block's column span, and windows of 16384 columns opened
+// greedily over its entries in stored order (a new window starts 4096
+// columns below the first column none of the open ones holds).  wb1 gets
+// the block's first column, wb4 up to four window bases (unused ones repeat
+// the first); flags bit 0: some block spans 65536 columns or more, bit 1:
+// some block needs more than four windows.
+constexpr int32_t kWindow = 16384;
+__global__ void short_analyse(const int32_t* __restrict__ colidx, const int2* __restrict__ blocks,
+                              int64_t nblocks, int32_t chunk, int32_t* __restrict__ wb1,
+                              int4* __restrict__ wb4, int* flags)
+{
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t bi = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; bi < nblocks;
+       bi += stride)
   {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess
-        || per_cu <= 0)
-      per_cu = 4;
-    it = occ.emplace(kernel, per_cu).first;
+    const int32_t k0 = blocks[bi].y, k1 = blocks[bi + 1].y;
+    int32_t lo = 0x7fffffff, hi = -1;
+    int32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+    int nw = 0;
+    const bool small = k1 - k0 <= chunk;
+    for (int32_t k = k0; small && k < k1; ++k)
+    {
+      const int32_t c = colidx[k];
+      lo = c < lo ? c : lo;
+      hi = c > hi ? c : hi;
+      const bool in = (nw > 0 && c >= b0 && c - b0 < kWindow) || (nw > 1 && c >= b1 && c - b1 < kWindow)
+                      || (nw > 2 && c >= b2 && c - b2 < kWindow)
+                      || (nw > 3 && c >= b3 && c - b3 < kWindow);
+      if (in)
+        continue;
+      const int32_t nb = c > 4096 ? c - 4096 : 0;
+      if (nw == 0)
+        b0 = nb;
+      else if (nw == 1)
+        b1 = nb;
+      else if (nw == 2)
+        b2 = nb;
+      else if (nw == 3)
+        b3 = nb;
+      ++nw;
+    }
+    if (nw < 2)
+      b1 = b0;
+    if (nw < 3)
+      b2 = b0;
+    if (nw < 4)
+      b3 = b0;
+    wb1[bi] = hi >= 0 ? lo : 0;
+    wb4[bi] = make_int4(b0, b1, b2, b3);
+    if (small && hi >= 0 && hi - lo >= 65536)
+      atomicOr(flags, 1);
+    if (small && nw > 4)
+      atomicOr(flags, 2);
   }
-  const int64_t g = static_cast<int64_t>(ncu) * it->second;
-  return static_cast<int>(g < nblocks ? g : nblocks);
 }
 
-int default_variant(int32_t dtype)
+// Encodes the entries of every block of at most `chunk` entries (one
+// workgroup per block): kind 1, the offset from the block's first column;
+// kind 2, (window << 14) | offset, the first window holding the column.
+// Kind 1 leaves the block's base in wbase.x.
+__global__ void short_encode(const int32_t* __restrict__ colidx, const int2* __restrict__ blocks,
+                             int64_t nblocks, int32_t chunk, int kind,
+                             const int32_t* __restrict__ wb1, int4* __restrict__ wbase,
+                             uint16_t* __restrict__ col16)
 {
-  // measured best CSR kernels on gfx950 for the stencil matrices: FP64
-  // 1024-thread workgroups staging 8192 entries (vec 2 x unroll 4), FP32 vec 4,
-  // BF16 vec 4 x unroll 2; SPMVX_SPMV_VARIANT overrides
+  for (int64_t bi = blockIdx.x; bi < nblocks; bi += gridDim.x)
+  {
+    const int32_t k0 = blocks[bi].y, k1 = blocks[bi + 1].y;
+    if (k1 - k0 > chunk)
+      continue;
+    const int4 w = wbase[bi];
+    const int32_t base = wb1[bi];
+    __syncthreads(); // every thread has read wbase[bi] before thread 0 rewrites it
+    for (int32_t k = k0 + static_cast<int32_t>(threadIdx.x); k < k1; k += blockDim.x)
+    {
+      const int32_t c = colidx[k];
+      if (kind == 1)
+      {
+        col16[k] = static_cast<uint16_t>(c - base);
+        continue;
+      }
+      uint32_t sel = 3, off = static_cast<uint32_t>(c - w.w);
+      if (c >= w.z && c - w.z < kWindow)
+      {
+        sel = 2;
+        off = static_cast<uint32_t>(c - w.z);
+      }
+      if (c >= w.y && c - w.y < kWindow)
+      {
+        sel = 1;
+        off = static_cast<uint32_t>(c - w.y);
+      }
+      if (c >= w.x && c - w.x < kWindow)
+      {
+        sel = 0;
+        off = static_cast<uint32_t>(c - w.x);
+      }
+      col16[k] = static_cast<uint16_t>((sel << 14) | off);
+    }
+    if (kind == 1 && threadIdx.x == 0)
+      wbase[bi] = make_int4(base, base, base, base);
+  }
+}
+
+} // namespace
+
+void build_short_index(Pattern& P, const RowBlocks& rbc, int chunk)
+{
+  std::lock_guard<std::recursive_mutex> lock(P.lazy_mu);
+  RowBlocks& rb = const_cast<RowBlocks&>(rbc); // owned by P, guarded by its lock
+  if (rb.short_built)
+    return;
+  rb.short_kind = 0;
+  if (rb.nblocks > 0 && P.nnz > 0)
+  {
+    DevBuf wb1, flags;
+    wb1.ensure(sizeof(int32_t) * static_cast<size_t>(rb.nblocks));
+    flags.ensure(sizeof(int));
+    rb.wbase.ensure(sizeof(int4) * static_cast<size_t>(rb.nblocks));
+    const int g = static_cast<int>(std::min<int64_t>((rb.nblocks + 255) / 256, 4096));
+    short_analyse<<<g, 256>>>(P.colidx.as<int32_t>(), rb.desc.as<int2>(), rb.nblocks, chunk,
+                              wb1.as<int32_t>(), rb.wbase.as<int4>(), flags.as<int>());
+    check_launch("short_analyse");
+    int f = 0;
+    SPMVX_HIP(hipMemcpy(&f, flags.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    const int kind = !(f & 1) ? 1 : (!(f & 2) ? 2 : 0);
+    if (kind)
+    {
+      rb.col16.ensure(sizeof(uint16_t) * static_cast<size_t>(P.nnz + 16));
+      const int ge = static_cast<int>(std::min<int64_t>(rb.nblocks, 65536));
+      short_encode<<<ge, 256>>>(P.colidx.as<int32_t>(), rb.desc.as<int2>(), rb.nblocks, chunk,
+                                kind, wb1.as<int32_t>(), rb.wbase.as<int4>(),
+                                rb.col16.as<uint16_t>());
+      check_launch("short_encode");
+      SPMVX_HIP(hipDeviceSynchronize());
+    }
+    else
+      rb.wbase.release();
+    rb.short_kind = kind;
+  }
+  rb.short_built = true;
+}
+
+CsrView csr_view(Pattern& P, int var)
+{
+  const RowBlocks& rb = variant_blocks(P, var);
+  const bool sh = variant_short(var);
+  if (sh)
+  {
+    build_short_index(P, rb, variant_block_nnz(var));
+    SPMVX_REQUIRE(rb.short_kind != 0 || P.nnz == 0,
+                  "the matrix does not fit 16-bit column indices (a row block's columns "
+                  "fall into more than four windows of 16384)");
+  }
+  return CsrView{P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(), rb.desc.as<int2>(), rb.nblocks,
+                 sh ? rb.col16.as<uint16_t>() : nullptr, sh ? rb.wbase.as<int4>() : nullptr,
+                 static_cast<int32_t>(P.ncols), sh ? rb.short_kind : 0};
+}
+
+CsrView csr_view_built(Pattern& P, int var)
+{
+  const RowBlocks* rb = built_variant_blocks(P, var);
+  const bool sh = variant_short(var);
+  if (!rb || (sh && !rb->short_built))
+    return CsrView{P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(), nullptr, 0, nullptr, nullptr,
+                   static_cast<int32_t>(P.ncols), 0};
+  return CsrView{P.rowptr.as<int32_t>(), P.colidx.as<int32_t>(), rb->desc.as<int2>(), rb->nblocks,
+                 sh ? rb->col16.as<uint16_t>() : nullptr, sh ? rb->wbase.as<int4>() : nullptr,
+                 static_cast<int32_t>(P.ncols), sh ? rb->short_kind : 0};
+}
+
+// Whether variant `var` of P can read 16-bit column indices (builds the
+// analysis).
+bool fits_short(Pattern& P, int var)
+{
+  const RowBlocks& rb = variant_blocks(P, var);
+  build_short_index(P, rb, variant_block_nnz(var));
+  return rb.short_kind != 0;
+}
+
+// Variant override from the environment (SPMVX_SPMV_VARIANT), -1 if none.
+int forced_variant()
+{
   static int forced = [] {
     const char* e = std::getenv("SPMVX_SPMV_VARIANT");
     if (!e)
@@ -216,11 +373,34 @@ int default_variant(int32_t dtype)
     const int x = std::atoi(e);
     return (x >= 0 && x < kNumVariants) ? x : -1;
   }();
-  if (forced >= 0 && dtype != SPMVX_C64 && dtype != SPMVX_C32)
-    return forced;
+  return forced;
+}
+
+int default_variant(int32_t dtype)
+{
+  // measured best 32-bit-index CSR kernels on gfx950 for the stencil
+  // matrices: FP64 1024-thread workgroups staging 8192 entries (vec 2 x
+  // unroll 4), FP32 vec 4, BF16 vec 4 x unroll 2; SPMVX_SPMV_VARIANT overrides
+  const int forced = forced_variant();
   if (dtype == SPMVX_C64 || dtype == SPMVX_C32) // variants 0-4 only (16 KB+ LDS products)
     return forced >= 0 && forced <= 4 ? forced : 2;
-  return dtype == SPMVX_F64 ? 11 : (dtype == SPMVX_F32 ? 3 : 4);
+  if (forced >= 0)
+    return forced;
+  return dtype == SPMVX_F64 ? 5 : (dtype == SPMVX_F32 ? 3 : 4);
+}
+
+int matrix_default_variant(spmvx_matrix_s* A)
+{
+  const int base = default_variant(A->dtype);
+  if (forced_variant() >= 0 || is_complex(A->dtype))
+    return base;
+  // the short-index counterpart of the default when the matrix fits it (the
+  // analysis of the inspector, Matrix.cpp:43)
+  const int sv = base == 5 ? 8 : (base == 3 ? 9 : (base == 4 ? 10 : -1));
+  Pattern& P = *A->pat;
+  if (sv < 0 || P.nnz == 0)
+    return base;
+  return fits_short(P, sv) ? sv : base;
 }
 
 namespace
@@ -260,7 +440,6 @@ __device__ __forceinline__ uint16_t dev_f64_to_bf16(double d)
   return static_cast<uint16_t>(u >> 16);
 }
 
-bool is_complex(int32_t dtype) { return dtype == SPMVX_C64 || dtype == SPMVX_C32; }
 
 
 __global__ void convert_values(const double* __restrict__ src, void* __restrict__ dst,
@@ -407,14 +586,12 @@ __global__ __launch_bounds__(kBS) void sddmm_kernel(const int32_t* __restrict__ 
 
 template <typename V, typename X, typename Acc>
 void launch_plain(int var, Pattern& P, const void* vals, const void* x, void* y, hipStream_t s,
-                  int32_t long_par)
+                  int32_t long_par, hipEvent_t ev0, hipEvent_t ev1)
 {
-  const RowBlocks& rb = variant_blocks(P, var);
-  launch_csr_stream<V, X, Acc, kPlain>(var, rb.nblocks, s, P.rowptr.as<int32_t>(),
-                                       P.colidx.as<int32_t>(), static_cast<const V*>(vals),
-                                       static_cast<const X*>(x), static_cast<X*>(y),
-                                       rb.desc.as<int2>(), nullptr, nullptr, nullptr, 0,
-                                       long_par);
+  const CsrView M = csr_view(P, var);
+  launch_csr_stream<V, X, Acc, kPlain>(var, M, s, static_cast<const V*>(vals),
+                                       static_cast<const X*>(x), static_cast<X*>(y), nullptr,
+                                       nullptr, nullptr, 0, long_par, ev0, ev1);
   check_launch("csr SpMV");
 }
 
@@ -433,8 +610,11 @@ __global__ void zero_words(uint32_t* __restrict__ w, int64_t n)
     w[i] = 0u;
 }
 
+// ev0 / ev1 (optional): events that record the product kernel's own start
+// and end (recorded around the zero fill when there are no entries).
 void launch_spmv(int32_t dtype, int var, Pattern& P, const void* vals, const void* x, void* y,
-                 hipStream_t s, int32_t long_par)
+                 hipStream_t s, int32_t long_par, hipEvent_t ev0 = nullptr,
+                 hipEvent_t ev1 = nullptr)
 {
   if (P.nnz == 0)
   {
@@ -442,23 +622,27 @@ void launch_spmv(int32_t dtype, int var, Pattern& P, const void* vals, const voi
     // columns without a branch, which needs x to hold at least one entry;
     // a matrix without entries may have no columns at all.
     const int64_t words = P.nrows * static_cast<int64_t>(vec_bytes(dtype)) / 4;
+    if (ev0)
+      SPMVX_HIP(hipEventRecord(ev0, s));
     if (words > 0)
     {
       zero_words<<<vec_grid(words), kVecBS, 0, s>>>(static_cast<uint32_t*>(y), words);
       check_launch("zero_words");
     }
+    if (ev1)
+      SPMVX_HIP(hipEventRecord(ev1, s));
     return;
   }
   if (dtype == SPMVX_F64)
-    launch_plain<double, double, double>(var, P, vals, x, y, s, long_par);
+    launch_plain<double, double, double>(var, P, vals, x, y, s, long_par, ev0, ev1);
   else if (dtype == SPMVX_F32)
-    launch_plain<float, float, float>(var, P, vals, x, y, s, long_par);
+    launch_plain<float, float, float>(var, P, vals, x, y, s, long_par, ev0, ev1);
   else if (dtype == SPMVX_BF16)
-    launch_plain<bf16_t, float, float>(var, P, vals, x, y, s, long_par);
+    launch_plain<bf16_t, float, float>(var, P, vals, x, y, s, long_par, ev0, ev1);
   else if (dtype == SPMVX_C64)
-    launch_plain<cplx<double>, cplx<double>, cplx<double>>(var, P, vals, x, y, s, 0);
+    launch_plain<cplx<double>, cplx<double>, cplx<double>>(var, P, vals, x, y, s, 0, ev0, ev1);
   else
-    launch_plain<cplx<float>, cplx<float>, cplx<float>>(var, P, vals, x, y, s, 0);
+    launch_plain<cplx<float>, cplx<float>, cplx<float>>(var, P, vals, x, y, s, 0, ev0, ev1);
 }
 
 // Fills A->svalues from A->values on stream s (layout built beforehand).
@@ -564,18 +748,25 @@ SlicedView sliced_view(spmvx_matrix_s* A)
 namespace
 {
 
-// CSR kernel variant used where the sliced layout does not apply
-// (transposed products).
-int csr_variant(const spmvx_matrix_s* A)
+// CSR kernel variant of the transposed product: the matrix's, unless it is
+// the sliced layout or a 16-bit-index kernel that the transposed pattern
+// does not fit (then the dtype's 32-bit default).
+int transposed_variant(spmvx_matrix_s* A)
 {
-  return A->variant == kSlicedVariant ? default_variant(A->dtype) : A->variant;
+  const int v = A->variant;
+  if (v == kSlicedVariant || (variant_short(v) && !fits_short(*A->tpat, v)))
+    return default_variant(A->dtype);
+  return v;
 }
 
-void launch_matrix_spmv(spmvx_matrix_s* A, const void* x, void* y, hipStream_t s)
+// ev0 / ev1 (optional): events that record the product kernel's own start
+// and end.
+void launch_matrix_spmv(spmvx_matrix_s* A, const void* x, void* y, hipStream_t s,
+                        hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
 {
   if (A->variant != kSlicedVariant)
   {
-    launch_spmv(A->dtype, A->variant, *A->pat, A->values.ptr, x, y, s, A->long_rows);
+    launch_spmv(A->dtype, A->variant, *A->pat, A->values.ptr, x, y, s, A->long_rows, ev0, ev1);
     return;
   }
   const SlicedView v = sliced_view(A);
@@ -583,17 +774,17 @@ void launch_matrix_spmv(spmvx_matrix_s* A, const void* x, void* y, hipStream_t s
     launch_sliced<double, double, double, kPlain>(v, s, A->svalues.as<double>(),
                                                   static_cast<const double*>(x),
                                                   static_cast<double*>(y), nullptr, nullptr,
-                                                  nullptr, 0);
+                                                  nullptr, 0, ev0, ev1);
   else if (A->dtype == SPMVX_F32)
     launch_sliced<float, float, float, kPlain>(v, s, A->svalues.as<float>(),
                                                static_cast<const float*>(x),
                                                static_cast<float*>(y), nullptr, nullptr, nullptr,
-                                               0);
+                                               0, ev0, ev1);
   else
     launch_sliced<bf16_t, float, float, kPlain>(v, s, A->svalues.as<bf16_t>(),
                                                 static_cast<const float*>(x),
                                                 static_cast<float*>(y), nullptr, nullptr, nullptr,
-                                                0);
+                                                0, ev0, ev1);
   check_launch("sliced SpMV");
 }
 
@@ -616,9 +807,8 @@ int spmvx_matrix_create(int32_t dtype, int64_t nrows, int64_t ncols, int64_t nnz
     SPMVX_REQUIRE(nnz == 0 || values != nullptr, "values is NULL");
     auto A = std::make_unique<spmvx_matrix_s>();
     A->dtype = dtype;
-    A->variant = default_variant(dtype);
     A->pat = upload_pattern(nrows, ncols, nnz, rowptr, colidx);
-    (void)variant_blocks(*A->pat, A->variant);
+    A->variant = matrix_default_variant(A.get());
     A->values.ensure(es * static_cast<size_t>(nnz + 16));
     if (nnz > 0)
     {
@@ -650,10 +840,9 @@ int spmvx_matrix_create_like(spmvx_matrix base, int32_t dtype, spmvx_matrix* out
     const size_t es = dtype_size(dtype);
     auto A = std::make_unique<spmvx_matrix_s>();
     A->dtype = dtype;
-    A->variant = default_variant(dtype);
     A->long_rows = base->long_rows;
     A->pat = base->pat;
-    (void)variant_blocks(*A->pat, A->variant);
+    A->variant = matrix_default_variant(A.get());
     const int64_t nnz = base->pat->nnz;
     A->values.ensure(es * static_cast<size_t>(nnz + 16));
     wait_values(base); // the conversion below runs on the null stream
@@ -728,10 +917,11 @@ int spmvx_matrix_set_kernel(spmvx_matrix A, int32_t variant)
     SPMVX_REQUIRE(variant >= -1 && variant <= kSlicedVariant, "unknown kernel variant");
     SPMVX_REQUIRE(!is_complex(A->dtype) || variant <= 4,
                   "kernel variant not available for complex values (use 0-4)");
-    const int v = variant < 0 ? default_variant(A->dtype) : variant;
+    const int v = variant < 0 ? matrix_default_variant(A) : variant;
     if (v == kSlicedVariant)
       (void)sliced_view(A); // builds the layout or reports why it does not apply
-    (void)variant_blocks(*A->pat, v); // row blocks built here, not at the first product
+    else
+      (void)csr_view(*A->pat, v); // row blocks (and short indices) built here
     A->variant = v;
   });
 }
@@ -760,14 +950,30 @@ int spmvx_matrix_get_kernel(spmvx_matrix A, int32_t* variant, char* name, int32_
     {
       std::string d;
       if (v == kSlicedVariant)
-        d = "sliced_spmv (row-pattern slices of 64 rows)";
+      {
+        // the slots each step loads (value loads x entries per load), as the
+        // launcher picks them from the padded pattern width
+        const SlicedView sv = sliced_view(A);
+        const int vb = static_cast<int>(dtype_size(A->dtype));
+        const int vw = 8 / vb;
+        const int wpad = (sv.wmax + vw - 1) / vw * vw;
+        d = "sliced_spmv (row-pattern slices of 64 rows, " + std::to_string(sv.rpl)
+            + " per wavefront, " + std::to_string(sliced_unr(vb, sv.rpl, wpad) * vw)
+            + " slots per step)";
+      }
       else
       {
         const SpmvVariant& k = kVariants[v];
-        d = std::string(k.kind == 0 ? "csr_stream" : k.kind == 1 ? "csr_pipe" : "csr_defer")
-            + " (" + std::to_string(k.bs)
-            + " threads, chunk " + std::to_string(k.chunk) + ", vec " + std::to_string(k.vec)
-            + ", unroll " + std::to_string(k.unroll) + (k.nt ? ", non-temporal" : "") + ")";
+        d = std::string("csr_stream (") + std::to_string(k.bs) + " threads, chunk "
+            + std::to_string(k.chunk) + ", vec " + std::to_string(k.vec) + ", unroll "
+            + std::to_string(k.unroll) + (k.nt ? ", non-temporal" : "")
+            + (k.i16 ? std::string(", 16-bit column indices, ")
+                           + (built_variant_blocks(*A->pat, v)
+                                      && built_variant_blocks(*A->pat, v)->short_kind == 1
+                                  ? "one base per block"
+                                  : "four windows per block")
+                     : std::string())
+            + ")";
       }
       const size_t n = std::min(d.size(), static_cast<size_t>(name_len - 1));
       std::memcpy(name, d.data(), n);
@@ -813,6 +1019,20 @@ int spmvx_spmv(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
   });
 }
 
+int spmvx_spmv_timed(spmvx_matrix A, const void* x, void* y, spmvx_stream stream,
+                     void* start_event, void* stop_event)
+{
+  return guarded([&] {
+    SPMVX_REQUIRE(A != nullptr, "NULL matrix");
+    SPMVX_REQUIRE(A->pat->nrows == 0 || y != nullptr, "y is NULL");
+    SPMVX_REQUIRE(A->pat->nnz == 0 || x != nullptr, "x is NULL");
+    SPMVX_REQUIRE(start_event != nullptr && stop_event != nullptr, "event is NULL");
+    require_disjoint(x, A->pat->ncols, y, A->pat->nrows, vec_bytes(A->dtype));
+    launch_matrix_spmv(A, x, y, as_stream(stream), static_cast<hipEvent_t>(start_event),
+                       static_cast<hipEvent_t>(stop_event));
+  });
+}
+
 int spmvx_spmv_t(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
 {
   return guarded([&] {
@@ -822,7 +1042,7 @@ int spmvx_spmv_t(spmvx_matrix A, const void* x, void* y, spmvx_stream stream)
     require_disjoint(x, A->pat->nrows, y, A->pat->ncols, vec_bytes(A->dtype));
     if (!A->tpat)
       build_transpose(A, as_stream(stream));
-    launch_spmv(A->dtype, csr_variant(A), *A->tpat, A->tvalues.ptr, x, y, as_stream(stream),
+    launch_spmv(A->dtype, transposed_variant(A), *A->tpat, A->tvalues.ptr, x, y, as_stream(stream),
                 A->long_rows);
   });
 }

@@ -35,7 +35,11 @@ def test_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert S.lib().spmvx_abi_version() == 1
+    """The loaded library reports the version the header declares."""
+    import re
+    hdr = open(os.path.join(os.path.dirname(GOLD), "..", "..", "include", "spmv_amd.h")).read()
+    want = int(re.search(r"#define SPMVX_ABI_VERSION (\d+)", hdr).group(1))
+    assert S.lib().spmvx_abi_version() == want == 2
 
 
 CASES = sorted({tuple(os.path.basename(p)[:-5].split("_")[:2])

@@ -373,3 +373,25 @@ def test_solvers_reject_null_vectors():
                              None) == 1
         assert b"NULL vector" in L.spmvx_last_error()
     assert its.value == -7
+
+
+def test_refinement_adaptive_inner_tolerance():
+    """SPMVX_IR_ADAPTIVE_INNER: each inner solve stops at the reduction the
+    outer residual still needs (never tighter than inner_rtol): the solve
+    still meets rtol in true residual, with no more inner iterations than
+    the fixed inner tolerance, and the solution agrees with FP64 CG."""
+    A = P.stencil(7, 48, 48, 48)
+    M64 = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    b = P.uniform(8, A.nrows, -1, 1)
+    bd = _dev(b)
+    x64, _ = S.cg(M64, bd, 10000, 1e-10)
+    for dt in ("f32", "bf16"):
+        Ml = M64.like(dt)
+        x0, st0 = S.ir_cg(M64, Ml, bd, rtol=1e-10, inner_rtol=1e-4)
+        x1, st1 = S.ir_cg(M64, Ml, bd, rtol=1e-10, inner_rtol=1e-4, adaptive_inner=True)
+        assert st1["rel_residual"] <= 1e-10 and _true_rel(A, x1.cpu().numpy(), b) <= 1.01e-10
+        assert st1["inner_its"] <= st0["inner_its"]
+        assert torch.linalg.norm(x1 - x64).item() <= 1e-7 * torch.linalg.norm(x64).item()
+    with pytest.raises(S.SpmvError, match="unknown refinement flags"):
+        S._check(S.lib().spmvx_ir_cg(M64.handle, Ml.handle, S._dptr(bd), S._dptr(x1), 4, 1e-10,
+                                     100, 1e-4, 2, None, None))

@@ -168,6 +168,33 @@ def test_spmv_full_size_27pt_ghost_plane():
     assert np.array_equal(y, O.spmv(_ocsr(A), x, threads=8))
 
 
+@pytest.mark.parametrize("variant", [-1, 5, S.SLICED_VARIANT])
+def test_spmv_timed_matches_product(variant):
+    """spmvx_spmv_timed: the same y as the product, and two events recorded by
+    the kernel itself (elapsed time positive and within the bracketing
+    events' span); without entries they bracket the zero fill."""
+    A = P.stencil(27, 24, 20, 22, ghost_planes=1)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    M.set_kernel(variant)
+    x = _dev(P.uniform(8, A.ncols, -1, 1))
+    y0 = M.mult(x)
+    y1 = torch.full_like(y0, 7.0)
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    s = torch.cuda.current_stream()
+    e[0].record(s)
+    M.mult_timed(x, y1, e[1], e[2])
+    e[3].record(s)
+    torch.cuda.synchronize()
+    assert torch.equal(y0, y1)
+    k, outer = e[1].elapsed_time(e[2]), e[0].elapsed_time(e[3])
+    assert 0 < k <= outer
+    Z = S.Matrix(np.zeros(4, np.int32), np.zeros(0, np.int32), np.zeros(0), ncols=2)
+    z = torch.full((3,), 5.0, dtype=torch.float64, device=DEV)
+    Z.mult_timed(torch.ones(2, dtype=torch.float64, device=DEV), z, e[1], e[2])
+    torch.cuda.synchronize()
+    assert torch.all(z == 0) and e[1].elapsed_time(e[2]) >= 0
+
+
 def test_spmv_empty_matrix():
     M = S.Matrix(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0), ncols=0)
     y = M.mult(torch.zeros(0, dtype=torch.float64, device=DEV))
@@ -400,6 +427,91 @@ def test_long_rows_cg_both_modes():
     assert abs(its[0] - its[1]) <= 2
 
 
+# ---------------------------------------------------------------- 16-bit column indices
+SHORT_VARIANTS = [8, 9, 10, 11]
+
+
+def _mixed_window_matrix(n=150000, seed=51):
+    """A band of width 41 around the diagonal, rows near n/3 that also reach
+    three far columns (their row blocks need four 16384-column windows, the
+    others one), empty rows, and one row longer than every staging chunk (a
+    block of its own, summed from the 32-bit indices)."""
+    rows = []
+    for i in range(n):
+        lo, hi = max(0, i - 20), min(n, i + 21)
+        cols = set(range(lo, hi))
+        if n // 3 <= i < n // 3 + 300:
+            cols |= {(i + 40000 * k) % n for k in range(1, 4)}
+        if i % 97 == 5:
+            cols = set()
+        if i == n // 2:
+            cols = set(range(0, n, 8))  # 18750 entries
+        rows.append(np.array(sorted(cols), np.int32))
+    rowptr = np.zeros(n + 1, np.int64)
+    rowptr[1:] = np.cumsum([len(r) for r in rows])
+    colidx = np.concatenate(rows)
+    return P.CSR(n, n, rowptr.astype(np.int32), colidx, P.uniform(seed, len(colidx), -1, 1))
+
+
+def test_short_index_variants_bitexact():
+    """The 16-bit column-index kernels decode the same columns in the same
+    order: FP64 products and transposed products bit-identical to the oracle,
+    FP32 / BF16 bit-identical to their 32-bit-index kernels, on a matrix whose
+    row blocks need one to four windows, with empty rows and a row longer
+    than the staging chunk."""
+    A = _mixed_window_matrix()
+    x = P.uniform(52, A.ncols, -1, 1)
+    want = O.spmv(_ocsr(A), x)
+    xt = P.uniform(53, A.nrows, -1, 1)
+    want_t = O.spmv_t(_ocsr(A), xt)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    xf = _dev(x, torch.float32)
+    ref = {dt: None for dt in ("f32", "bf16")}
+    for dt in ref:
+        Ml = M.like(dt)
+        Ml.set_kernel(3 if dt == "f32" else 4)
+        ref[dt] = Ml.mult(xf).cpu().numpy()
+    for v in SHORT_VARIANTS:
+        M.set_kernel(v)
+        assert "16-bit" in M.kernel_name() and "windows" in M.kernel_name()
+        assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), want), v
+        assert np.array_equal(M.transpmult(_dev(xt)).cpu().numpy(), want_t), v
+        for dt in ref:
+            Ml = M.like(dt)
+            Ml.set_kernel(v)
+            assert np.array_equal(Ml.mult(xf).cpu().numpy(), ref[dt]), (v, dt)
+
+
+def test_short_index_defaults():
+    """Stencil and band matrices get a 16-bit-index kernel by default (FP64,
+    FP32 and BF16 values): one base per row block when a block's columns span
+    less than 65536, else up to four 16384-column windows per block (the
+    7-point operator on 192 x 192 x 20, whose plane neighbours are 36864
+    columns away: three windows); a matrix with uniformly random columns over
+    a wide range keeps 32-bit indices, as do complex values."""
+    for A, kind in ((P.stencil(27, 20, 20, 20, ghost_planes=1), "one base"),
+                    (P.stencil(7, 192, 192, 20), "windows"),
+                    (P.stencil(5, 200, 200), "one base")):
+        M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+        assert "16-bit" in M.kernel_name() and kind in M.kernel_name()
+        for dt in ("f32", "bf16"):
+            assert "16-bit" in M.like(dt).kernel_name()
+        x = P.uniform(54, A.ncols, -1, 1)
+        assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), O.spmv(_ocsr(A), x))
+        xt = P.uniform(55, A.nrows, -1, 1)
+        assert np.array_equal(M.transpmult(_dev(xt)).cpu().numpy(), O.spmv_t(_ocsr(A), xt))
+    # uniformly random columns over 200000: no block fits four 16384-column
+    # windows
+    R = P.random_spd(200000, 3)
+    MR = S.Matrix(R.rowptr, R.colidx, R.val, ncols=R.ncols)
+    assert "16-bit" not in MR.kernel_name()
+    with pytest.raises(S.SpmvError, match="16-bit"):
+        MR.set_kernel(SHORT_VARIANTS[0])
+    C = P.stencil(7, 8, 8, 8)
+    assert "16-bit" not in S.Matrix(C.rowptr, C.colidx, C.val.astype(complex), ncols=C.ncols,
+                                    dtype="c64").kernel_name()
+
+
 # ---------------------------------------------------------------- sliced row-pattern layout
 SLICED_FIT = ["demo1d", "poisson5", "poisson7", "stencil27_ghost", "diffusion", "one_row"]
 
@@ -433,17 +545,35 @@ def band_matrix(n, lo, hi, seed):
     return P.CSR(n, n, rowptr, colidx, P.uniform(seed, len(colidx), -1, 1))
 
 
-@pytest.mark.parametrize("lo,hi", [(2, 2), (4, 4), (7, 7), (8, 7), (8, 8), (14, 14), (20, 20)],
-                         ids=["w5", "w9", "w15", "w16", "w17", "w29", "w41"])
-def test_sliced_layout_pattern_widths(lo, hi):
-    """Pattern widths on both sides of the kernels' per-step slot counts
-    (FP64: 4 up to width 4, 8 up to 16, 14 above, one to three steps with
-    idle slots; FP32: 2 or 4 slots, BF16: 4, 8 or 12 slots by the padded
-    width): FP64 bit-identical to the oracle, FP32/BF16 to the CSR kernels."""
+SLICED_WIDTHS = [(1, 1), (2, 2), (4, 4), (7, 7), (8, 7), (8, 8), (14, 14), (20, 20)]
+# slots per step the launcher picks for each width above (fewest idle slots,
+# then fewest steps, among the compiled steps): FP64 4/8/10/14, FP32 2/4,
+# BF16 4/8/12
+SLICED_STEPS = {"f64": [4, 8, 10, 8, 8, 10, 10, 14], "f32": [4, 2, 2, 4, 4, 2, 2, 2],
+                "bf16": [4, 8, 12, 8, 8, 4, 8, 4]}
+_steps_seen = {dt: set() for dt in SLICED_STEPS}
+
+
+def _slots_per_step(M):
+    import re
+    return int(re.search(r"(\d+) slots per step", M.kernel_name()).group(1))
+
+
+@pytest.mark.parametrize("case", range(len(SLICED_WIDTHS)),
+                         ids=["w3", "w5", "w9", "w15", "w16", "w17", "w29", "w41"])
+def test_sliced_layout_pattern_widths(case):
+    """Band matrices of pattern widths 3-41 on both sides of the sliced
+    kernels' per-step slot counts: the step the launcher picks (reported by
+    the kernel description) is the expected one, FP64 is bit-identical to the
+    oracle and FP32/BF16 to the CSR kernels; over the cases every compiled
+    step of every value type is used."""
+    lo, hi = SLICED_WIDTHS[case]
     A = band_matrix(3001, lo, hi, seed=lo * 100 + hi)
     M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
     M.set_kernel(S.SLICED_VARIANT)
     assert M.sliced_info()["wmax"] == lo + hi + 1
+    assert _slots_per_step(M) == SLICED_STEPS["f64"][case]
+    _steps_seen["f64"].add(_slots_per_step(M))
     x = P.uniform(23, A.ncols, -1, 1)
     assert np.array_equal(M.mult(_dev(x)).cpu().numpy(), O.spmv(_ocsr(A), x))
     xf = _dev(x, torch.float32)
@@ -451,7 +581,16 @@ def test_sliced_layout_pattern_widths(lo, hi):
         Ml = M.like(dt)
         ref = Ml.mult(xf).cpu().numpy()
         Ml.set_kernel(S.SLICED_VARIANT)
+        assert _slots_per_step(Ml) == SLICED_STEPS[dt][case], dt
+        _steps_seen[dt].add(_slots_per_step(Ml))
         assert np.array_equal(Ml.mult(xf).cpu().numpy(), ref), dt
+
+
+def test_sliced_layout_every_step_used():
+    """The width cases above cover every compiled step (runs after them)."""
+    if not all(_steps_seen.values()):
+        pytest.skip("runs after the width cases")
+    assert _steps_seen == {"f64": {4, 8, 10, 14}, "f32": {2, 4}, "bf16": {4, 8, 12}}
 
 
 def test_sliced_layout_info():
@@ -610,8 +749,8 @@ def test_create_like_copies_from_two_threads():
     M32 = M64.like("f32")
     xd, xfd = _dev(x), _dev(xf, torch.float32)
     torch.cuda.synchronize()  # inputs on the device before the side streams read them
-    t1 = threading.Thread(target=work, args=(M64, xd, want64, [0, 5, 8, 11, 14, 13, 9]))
-    t2 = threading.Thread(target=work, args=(M32, xfd, want32, [12, 3, 7, 10, 1, 14, 6]))
+    t1 = threading.Thread(target=work, args=(M64, xd, want64, [0, 5, 8, 11, 12, 7, 9]))
+    t2 = threading.Thread(target=work, args=(M32, xfd, want32, [12, 3, 7, 10, 1, 6, 2]))
     t1.start()
     t2.start()
     t1.join()
