@@ -254,6 +254,32 @@ int spmvx_cg(spmvx_matrix A, spmvx_l2gmap col_map, const double* b_dev, double* 
              int32_t max_its, double rtol, int32_t* its, double* hist_host,
              spmvx_stream stream);
 
+/*
+ * spmvx_cg without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * checks and results as spmvx_cg (the same kernels in the same order: x, *its
+ * and the history are bit-identical), but every buffer of the solve (r, A p,
+ * the direction vector, partial sums, the device-side scalars and the
+ * history) lives in a fixed device array of SPMVX_CG_NOALLOC_SCRATCH_BYTES
+ * that the library holds on every GPU it runs on until the process ends, and
+ * the kernels are launched directly on `stream` (no captured graph, no
+ * internal stream or event).  The call needs about 8 * (2 * nrows + ncols)
+ * bytes plus 8 * (max_its + 1) with a history; a solve that needs more is
+ * rejected with SPMVX_ERR_ARG before any kernel runs.  Preconditions (else
+ * SPMVX_ERR_ARG, nothing launched): the row blocks of A's kernel variant and,
+ * for the sliced variant, its layout exist (spmvx_matrix_create and
+ * spmvx_matrix_set_kernel build them), and a col_map with ghosts or pack
+ * indices has been used once (spmvx_l2gmap_update or spmvx_cg), which builds
+ * its device plan.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two host threads, or two streams in flight) race and
+ * must be serialised by the caller.  Synchronises `stream`.
+ */
+#define SPMVX_CG_NOALLOC_SCRATCH_BYTES (32u << 20)
+int spmvx_cg_noalloc(spmvx_matrix A, spmvx_l2gmap col_map, const double* b_dev, double* x_dev,
+                     int32_t max_its, double rtol, int32_t* its, double* hist_host,
+                     spmvx_stream stream);
+
 /* ===================================================================== */
 /* PETSc binary input (spmv/read_petsc.h:10-31, spmv/read_petsc.cpp)      */
 /* ===================================================================== */

@@ -23,7 +23,7 @@ import os
 
 import numpy as np
 
-__all__ = ["L2GMap", "Matrix", "Batch", "cg", "cg_f32", "ir_cg", "sddmm", "copy_f64", "read_f64", "SpmvError",
+__all__ = ["L2GMap", "Matrix", "Batch", "cg", "cg_noalloc", "cg_f32", "ir_cg", "sddmm", "copy_f64", "read_f64", "SpmvError",
            "read_petsc_csr", "read_petsc_matrix", "read_petsc_vector",
            "F64", "F32", "BF16", "C64", "C32", "SLICED_VARIANT", "lib", "library_path"]
 
@@ -105,6 +105,7 @@ def lib():
         "spmvx_spmv_t": ([P, P, P, P], ctypes.c_int),
         "spmvx_spmv_timed": ([P, P, P, P, P, P], ctypes.c_int),
         "spmvx_cg": ([P, P, P, P, i32, f64, P, P, P], ctypes.c_int),
+        "spmvx_cg_noalloc": ([P, P, P, P, i32, f64, P, P, P], ctypes.c_int),
         "spmvx_cg_f32": ([P, P, P, i32, f64, P, P], ctypes.c_int),
         "spmvx_ir_cg": ([P, P, P, P, i32, f64, i32, f64, i32, P, P], ctypes.c_int),
         "spmvx_sddmm": ([P, f64, P, P, P, P], ctypes.c_int),
@@ -459,6 +460,31 @@ def cg(A, b, max_its, rtol, history=False, stream=None):
     _check(lib().spmvx_cg(A.handle, cm.handle if cm is not None else None, _dptr(b), _dptr(x),
                           int(max_its), float(rtol), ctypes.byref(its), _np_ptr(hist),
                           _stream(stream)))
+    if history:
+        return x, its.value, hist[: its.value + 1]
+    return x, its.value
+
+
+def cg_noalloc(A, b, max_its, rtol, x=None, history=False, stream=None):
+    """spmvx_cg_noalloc: spmv::cg with every buffer of the solve in the
+    library's fixed device scratch (no allocation by the library; x may be
+    given so that the caller allocates nothing either).  Returns (x, k) or
+    (x, k, hist) like cg()."""
+    import torch
+    if b.numel() != A.nrows:
+        raise SpmvError("spmv::cg - Error: b.rows() != A.rows()")
+    if b.dtype != torch.float64:
+        raise SpmvError("b must be float64")
+    if x is None:
+        x = torch.empty(A.ncols, dtype=torch.float64, device=b.device)
+    elif x.numel() != A.ncols or x.dtype != torch.float64:
+        raise SpmvError("x must hold A.ncols float64 values")
+    its = ctypes.c_int32()
+    hist = np.zeros(max_its + 1, np.float64) if history else None
+    cm = A.col_map()
+    _check(lib().spmvx_cg_noalloc(A.handle, cm.handle if cm is not None else None, _dptr(b),
+                                  _dptr(x), int(max_its), float(rtol), ctypes.byref(its),
+                                  _np_ptr(hist), _stream(stream)))
     if history:
         return x, its.value, hist[: its.value + 1]
     return x, its.value

@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "handles.hpp"
 
@@ -222,7 +224,155 @@ __global__ void add_correction(double* __restrict__ x, const float* __restrict__
     x[i] += norm * static_cast<double>(d[i]);
 }
 
-// Argument checks of spmvx_cg.
+// ---------------------------------------------------------------- no-allocation CG
+// Scratch of spmvx_cg_noalloc: a file-scope device array, one copy on every
+// GPU the library runs on, held by the loaded code object until the process
+// ends.  In host code `scratch` names a host-side stand-in, not device
+// memory: its address on the current device comes from hipGetSymbolAddress,
+// looked up once per device and kept in scratch_addr.
+constexpr size_t kNoAllocBytes = SPMVX_CG_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void* scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device.
+unsigned char* noalloc_scratch()
+{
+  int dev = 0;
+  SPMVX_HIP(hipGetDevice(&dev));
+  SPMVX_REQUIRE(dev >= 0 && dev < kMaxDevices, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_addr[dev])
+  {
+    void* p = nullptr;
+    SPMVX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SPMVX_REQUIRE(p != nullptr, "no device address for the CG scratch");
+    scratch_addr[dev] = p;
+  }
+  return static_cast<unsigned char*>(scratch_addr[dev]);
+}
+
+// Offsets of the CG buffers inside the scratch (256-byte aligned).
+struct NoAllocLayout
+{
+  size_t r = 0, y = 0, p = 0, part_s = 0, part_0 = 0, part_1 = 0, state = 0, hist = 0, total = 0;
+};
+
+NoAllocLayout noalloc_layout(int64_t n, int64_t ncols, int64_t nparts, int vgrid,
+                             int64_t hist_n)
+{
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  NoAllocLayout L;
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    const size_t o = off;
+    off += al(b);
+    return o;
+  };
+  L.r = take(8 * static_cast<size_t>(n > 0 ? n : 1));
+  L.y = take(8 * static_cast<size_t>(n > 0 ? n : 1));
+  L.p = take(8 * static_cast<size_t>(ncols > 0 ? ncols : 1));
+  L.part_s = take(8 * static_cast<size_t>(nparts > 0 ? nparts : 1));
+  L.part_0 = take(8 * static_cast<size_t>(vgrid));
+  L.part_1 = take(8 * static_cast<size_t>(vgrid));
+  L.state = take(sizeof(CgState<double>));
+  L.hist = take(8 * static_cast<size_t>(hist_n > 0 ? hist_n : 1));
+  L.total = off;
+  return L;
+}
+
+// FP64 CG with every buffer in the scratch: the kernels of run_cg, launched
+// directly on the caller's stream (no captured graph, no workspace), the stop
+// flag read after every chunk of iterations with a copy into host memory of
+// the caller's stack.
+int run_cg_noalloc(spmvx_matrix_s* A, spmvx_l2gmap_s* col_map, const double* b, double* x,
+                   int32_t kmax, double rtol, double* hist_host, hipStream_t s)
+{
+  Pattern& P = *A->pat;
+  const bool sliced = A->variant == kSlicedVariant;
+  const CsrView rbv = csr_view_built(P, A->variant);
+  const CsrView* rb = rbv.blocks ? &rbv : nullptr;
+  SPMVX_REQUIRE(sliced || rb != nullptr,
+                "the row blocks of the matrix's kernel variant are not built");
+  SPMVX_REQUIRE(!sliced || A->svalues_ready, "the sliced layout of the matrix is not built");
+  if (col_map)
+  {
+    const bool needs_plan = !col_map->indexbuf.empty() || !col_map->ghosts.empty();
+    SPMVX_REQUIRE(!needs_plan || col_map->device_ready,
+                  "the column map has no device plan yet: call spmvx_l2gmap_update (or "
+                  "spmvx_cg) with it once first");
+    SPMVX_REQUIRE(col_map->halo_dtype < 0 || col_map->halo_dtype == SPMVX_F64,
+                  "halo dtype differs from the vector dtype");
+  }
+  const int64_t n = P.nrows, ncols = P.ncols;
+  const int vg = vec_grid(n);
+  const int g0 = vec_grid(ncols > n ? ncols : n);
+  const int64_t hist_n = hist_host ? static_cast<int64_t>(kmax) + 1 : 0;
+  const SlicedView sv = sliced ? sliced_view(A) : SlicedView{}; // built: no allocation
+  const int64_t spw = (kSliceBS / kSlice) * (sliced ? sv.rpl : 1);
+  const int64_t nparts_max = sliced ? (sv.nslices + spw - 1) / spw : rb->nblocks;
+  const NoAllocLayout L = noalloc_layout(n, ncols, nparts_max, g0, hist_n);
+  if (L.total > kNoAllocBytes)
+    throw Error(SPMVX_ERR_ARG, "spmvx_cg_noalloc: the solve needs " + std::to_string(L.total)
+                                   + " bytes of scratch, more than its "
+                                   + std::to_string(kNoAllocBytes) + " bytes");
+  unsigned char* base = noalloc_scratch();
+  double* r = reinterpret_cast<double*>(base + L.r);
+  double* y = reinterpret_cast<double*>(base + L.y);
+  double* p = reinterpret_cast<double*>(base + L.p);
+  double* part_s = reinterpret_cast<double*>(base + L.part_s);
+  double* part_0 = reinterpret_cast<double*>(base + L.part_0);
+  double* part_1 = reinterpret_cast<double*>(base + L.part_1);
+  auto* st = reinterpret_cast<CgState<double>*>(base + L.state);
+  double* hist_dev = hist_host ? reinterpret_cast<double*>(base + L.hist) : nullptr;
+  const void* vals = sliced ? A->svalues.ptr : A->values.ptr;
+
+  // the iteration of run_cg (cg.cpp:52-80), every kernel gated by the state
+  auto iteration = [&]() {
+    if (col_map)
+      l2gmap_update_async(col_map, SPMVX_F64, p, s); // cg.cpp:55
+    const int64_t nparts =
+        sliced ? launch_sliced<double, double, double, kDot>(
+                     sv, s, static_cast<const double*>(vals), p, y, nullptr, part_s, st, 1)
+               : launch_csr_stream<double, double, double, kDot>(
+                     A->variant, *rb, s, static_cast<const double*>(vals), p, y, nullptr, part_s,
+                     st, 1, A->long_rows);
+    cg_alpha_finalize<double><<<1, kFinBS, 0, s>>>(part_s, nparts, st, 1);
+    cg_update_r<double, double><<<vg, kVecBS, 0, s>>>(n, r, y, part_1, st, 1);
+    cg_beta_finalize<double><<<1, kFinBS, 0, s>>>(part_1, vg, st, hist_dev, 1);
+    cg_update_xp<double, double><<<vg, kVecBS, 0, s>>>(n, x, p, r, st, 1);
+  };
+
+  cg_state_init<double><<<1, 1, 0, s>>>(st, kmax, rtol * rtol);
+  cg_init<double, double><<<g0, kVecBS, 0, s>>>(n, ncols, b, x, r, p, part_0);
+  cg_init_finalize<double><<<1, kFinBS, 0, s>>>(part_0, g0, st, hist_dev);
+  check_launch("cg_init");
+  int32_t launched = 0;
+  int32_t stopped = kmax <= 0 ? 1 : 0;
+  while (!stopped && launched < kmax)
+  {
+    for (int j = 0; j < kGraphChunk; ++j)
+      iteration();
+    check_launch("cg iteration");
+    launched += kGraphChunk;
+    SPMVX_HIP(hipMemcpyAsync(&stopped, &st->stopped, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SPMVX_HIP(hipStreamSynchronize(s));
+  }
+  CgState<double> fin{};
+  SPMVX_HIP(hipMemcpyAsync(&fin, st, sizeof(fin), hipMemcpyDeviceToHost, s));
+  SPMVX_HIP(hipStreamSynchronize(s));
+  if (hist_host)
+  {
+    SPMVX_HIP(hipMemcpyAsync(hist_host, hist_dev, sizeof(double) * (static_cast<size_t>(fin.k) + 1),
+                             hipMemcpyDeviceToHost, s));
+    SPMVX_HIP(hipStreamSynchronize(s));
+  }
+  return fin.k;
+}
+
+// Argument checks shared by spmvx_cg and spmvx_cg_noalloc.
 void check_cg_args(spmvx_matrix A, spmvx_l2gmap col_map, const double* b, const double* x,
                    int32_t max_its)
 {
@@ -267,6 +417,18 @@ int spmvx_cg(spmvx_matrix A, spmvx_l2gmap col_map, const double* b, double* x, i
     if (hist_host)
       SPMVX_HIP(hipMemcpy(hist_host, hd, sizeof(double) * (static_cast<size_t>(k) + 1),
                           hipMemcpyDeviceToHost));
+  });
+}
+
+int spmvx_cg_noalloc(spmvx_matrix A, spmvx_l2gmap col_map, const double* b, double* x,
+                     int32_t max_its, double rtol, int32_t* its, double* hist_host,
+                     spmvx_stream stream)
+{
+  return guarded([&] {
+    check_cg_args(A, col_map, b, x, max_its);
+    const int k = run_cg_noalloc(A, col_map, b, x, max_its, rtol, hist_host, as_stream(stream));
+    if (its)
+      *its = k;
   });
 }
 

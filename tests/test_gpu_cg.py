@@ -368,6 +368,8 @@ def test_solvers_reject_null_vectors():
     bp = ctypes.c_void_p(b.data_ptr())
     for bb, xx in ((None, bp), (bp, None)):
         assert L.spmvx_cg(M64.handle, None, bb, xx, 10, 1e-10, ctypes.byref(its), None, None) == 1
+        assert L.spmvx_cg_noalloc(M64.handle, None, bb, xx, 10, 1e-10, ctypes.byref(its), None,
+                                  None) == 1
         assert L.spmvx_cg_f32(M32.handle, bb, xx, 10, 1e-6, ctypes.byref(its), None) == 1
         assert L.spmvx_ir_cg(M64.handle, M32.handle, bb, xx, 4, 1e-10, 100, 1e-4, 0, None,
                              None) == 1
