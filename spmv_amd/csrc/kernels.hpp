@@ -345,23 +345,33 @@ __global__ __launch_bounds__(BS) void csr_stream(
     const int2* __restrict__ blocks, int64_t nblocks, const X* __restrict__ b,
     Acc* __restrict__ partials, const CgState<Acc>* st, int32_t it, int32_t long_par)
 {
-  if (MODE != kPlain && !cg_live(st, it))
-    return;
   __shared__ Acc prod[CHUNK + 1];
   __shared__ Acc red[16];
 
   const int64_t bi = xcd_block(blockIdx.x, nblocks);
   const int2 d0 = blocks[bi], d1 = blocks[bi + 1];
+  // A stopped CG iteration returns once both the state and the block
+  // descriptor have arrived (the test reads both), so that the two loads are
+  // in flight together rather than one after the other.
+  if constexpr (MODE != kPlain)
+    if (!cg_live(st, it) & (d1.y >= d0.y))
+      return;
   const int32_t r0 = d0.x, r1 = d1.x;
   const int32_t k0 = d0.y, k1 = d1.y;
   const int tid = threadIdx.x;
   const int32_t row = r0 + tid;
-  // row extents, loaded early (independent of the staging below)
+  // row extents and the row's x (p.y) or b (residual) entry, loaded early
+  // (independent of the staging below)
   int32_t ra = 0, re = 0;
+  X xr = X(0);
   if (row < r1)
   {
     ra = rowptr[row];
     re = rowptr[row + 1];
+    if constexpr (MODE == kDot)
+      xr = x[row];
+    else if constexpr (MODE == kResid)
+      xr = b[row];
   }
   Acc part = Acc(0);
   int4 w = make_int4(0, 0, 0, 0); // base / window bases of a short-index block
@@ -440,7 +450,7 @@ __global__ __launch_bounds__(BS) void csr_stream(
       const Acc s = row_sum(prod, ra - k0, re - k0);
       if (MODE == kResid)
       {
-        const X yi = b[row] - static_cast<X>(s);
+        const X yi = xr - static_cast<X>(s);
         store_result(y + row, yi);
         part = static_cast<Acc>(yi) * static_cast<Acc>(yi);
       }
@@ -449,7 +459,7 @@ __global__ __launch_bounds__(BS) void csr_stream(
         const X yi = static_cast<X>(s);
         store_result(y + row, yi);
         if (MODE == kDot)
-          part = static_cast<Acc>(x[row]) * static_cast<Acc>(yi);
+          part = static_cast<Acc>(xr) * static_cast<Acc>(yi);
       }
     }
   }
@@ -622,11 +632,19 @@ constexpr int kSlice = 64;
 constexpr int kSlicedVariant = SPMVX_SLICED_VARIANT;  // kernel variant id of this layout
 static_assert(kSlicedVariant == kNumVariants, "the sliced layout follows the CSR variants");
 
-// Entries per lane per value load: 8-byte loads for every value type.
+// Entries per lane per value load (VW) for values of vbytes bytes: one FP64
+// or FP32 value (no padding: the slice width stays the longest row), four
+// BF16 values (an 8-byte load).  Measured on the 7- and 27-point stencils:
+// FP32 with 4-byte loads and no padding ran 12-18% faster than with 8-byte
+// loads of value pairs (7 entries padded to 8).
+__host__ __device__ constexpr int sliced_vw_of(int vbytes)
+{
+  return vbytes == 2 ? 4 : 1;
+}
 template <typename V>
 constexpr int sliced_vw()
 {
-  return static_cast<int>(8 / sizeof(V));
+  return sliced_vw_of(static_cast<int>(sizeof(V)));
 }
 
 // One wavefront handles RPL consecutive slices (64 x RPL rows, lane l owns
@@ -648,19 +666,40 @@ __global__ __launch_bounds__(kSliceBS) void sliced_spmv(
   constexpr int SPW = (kSliceBS / kSlice) * RPL; // slices per workgroup
   extern __shared__ int32_t sdict[];             // dict_n entries
   __shared__ Acc red[16];
-  for (int i = threadIdx.x; i < dict_n; i += kSliceBS)
-    sdict[i] = dict[i];
   const int64_t nblk = (nslices + SPW - 1) / SPW;
   const int64_t bi = xcd_block(blockIdx.x, nblk);
   const int64_t s0 = bi * SPW + (threadIdx.x >> 6) * RPL; // this wavefront's first slice
   const int lane = threadIdx.x & 63;
+  // slice descriptors, pattern ids and the row's x (p.y) or b (residual)
+  // entry are loaded before the pattern table is staged, so that they are in
+  // flight while the workgroup waits for the table
+  int2 d[RPL];
+  int p[RPL];
+  bool has[RPL];
+  X xr[RPL];
+#pragma unroll
+  for (int j = 0; j < RPL; ++j)
+  {
+    const int64_t sl = s0 + j;
+    const bool sv = sl < nslices;
+    d[j] = sv ? sdesc[sl] : make_int2(0, 0); // W = 0: loads stay at slot 0
+    const int64_t row = sl * kSlice + lane;
+    has[j] = sv && row < nrows;
+    p[j] = has[j] ? static_cast<int>(pid[row]) : 0;
+    xr[j] = X(0);
+    if constexpr (MODE == kDot)
+      xr[j] = has[j] ? x[row] : X(0);
+    else if constexpr (MODE == kResid)
+      xr[j] = has[j] ? b[row] : X(0);
+  }
+  for (int i = threadIdx.x; i < dict_n; i += kSliceBS)
+    sdict[i] = dict[i];
   lds_barrier();
   Acc part = Acc(0);
   if (s0 < nslices) // uniform per wavefront
   {
     int W = 0;
     int wj[RPL], len[RPL];
-    bool has[RPL];
     const int32_t* pat[RPL];
     const V* vs[RPL];
     int32_t r32[RPL], rc[RPL];
@@ -669,19 +708,15 @@ __global__ __launch_bounds__(kSliceBS) void sliced_spmv(
     for (int j = 0; j < RPL; ++j)
     {
       const int64_t sl = s0 + j;
-      const bool sv = sl < nslices;
-      const int2 d = sv ? sdesc[sl] : make_int2(0, 0); // W = 0: loads stay at slot 0
-      wj[j] = d.y;
-      W = d.y > W ? d.y : W;
+      wj[j] = d[j].y;
+      W = d[j].y > W ? d[j].y : W;
       const int64_t row = sl * kSlice + lane;
-      has[j] = sv && row < nrows;
-      const int p = has[j] ? static_cast<int>(pid[row]) : 0;
-      pat[j] = sdict + p * (wmax + 1);
+      pat[j] = sdict + p[j] * (wmax + 1);
       len[j] = has[j] ? pat[j][0] : 0;
       // column used by idle slots: any valid index (ncols >= 1 when nnz >= 1)
       rc[j] = static_cast<int32_t>(row < ncols ? row : ncols - 1);
       r32[j] = static_cast<int32_t>(row);
-      vs[j] = val + static_cast<int64_t>(d.x) * kSlice + lane * VW;
+      vs[j] = val + static_cast<int64_t>(d[j].x) * kSlice + lane * VW;
       sum[j] = Acc(0);
     }
     for (int k0 = 0; k0 < W; k0 += VW * UNR)
@@ -728,7 +763,7 @@ __global__ __launch_bounds__(kSliceBS) void sliced_spmv(
         const int64_t row = (s0 + j) * kSlice + lane;
         if (MODE == kResid)
         {
-          const X yi = b[row] - static_cast<X>(sum[j]);
+          const X yi = xr[j] - static_cast<X>(sum[j]);
           store_result(y + row, yi);
           part += static_cast<Acc>(yi) * static_cast<Acc>(yi);
         }
@@ -737,7 +772,7 @@ __global__ __launch_bounds__(kSliceBS) void sliced_spmv(
           const X yi = static_cast<X>(sum[j]);
           store_result(y + row, yi);
           if (MODE == kDot)
-            part += static_cast<Acc>(x[row]) * static_cast<Acc>(yi);
+            part += static_cast<Acc>(xr[j]) * static_cast<Acc>(yi);
         }
       }
   }
@@ -785,20 +820,22 @@ inline int sliced_step(int w, std::initializer_list<int> sizes)
 // each value type's default rpl, the compiled step (UNR x VW slots) that
 // leaves the fewest idle slots, then the fewest steps (the best of the sizes
 // compiled on each of the 27- and 7-point stencils and banded matrices of
-// widths 5-41 measured); with rpl set by hand, the fixed step.
+// widths 5-41 measured; a step of exactly 7 entries took the 7-point
+// stencil's FP64 product from 281 to 260 us and its FP32 one from 118 to
+// 114 us); with rpl set by hand, the fixed step.
 inline int sliced_unr(int vbytes, int rpl, int wpad)
 {
   if (vbytes == 8)
   {
     if (rpl == 1)
-      return sliced_step(wpad, {4, 8, 10, 14});
+      return sliced_step(wpad, {4, 7, 8, 10, 14});
     return 4;
   }
   if (vbytes == 4)
   {
-    if (rpl == 4)
-      return sliced_step(wpad, {2, 4}) / 2;
-    return 2;
+    if (rpl == 2)
+      return sliced_step(wpad, {4, 7, 8, 14});
+    return 4;
   }
   if (rpl == 2)
     return sliced_step(wpad, {4, 8, 12}) / 4;
@@ -836,6 +873,8 @@ int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X*
         go(integral_constant<int, 10>{}, integral_constant<int, 1>{});
       else if (u == 8)
         go(integral_constant<int, 8>{}, integral_constant<int, 1>{});
+      else if (u == 7)
+        go(integral_constant<int, 7>{}, integral_constant<int, 1>{});
       else
         go(integral_constant<int, 4>{}, integral_constant<int, 1>{});
     }
@@ -846,17 +885,21 @@ int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X*
   }
   else if constexpr (sizeof(V) == 4)
   {
-    if (S.rpl == 4)
+    if (S.rpl == 2)
     {
-      if (u == 2)
-        go(integral_constant<int, 2>{}, integral_constant<int, 4>{});
+      if (u == 14)
+        go(integral_constant<int, 14>{}, integral_constant<int, 2>{});
+      else if (u == 8)
+        go(integral_constant<int, 8>{}, integral_constant<int, 2>{});
+      else if (u == 7)
+        go(integral_constant<int, 7>{}, integral_constant<int, 2>{});
       else
-        go(integral_constant<int, 1>{}, integral_constant<int, 4>{});
+        go(integral_constant<int, 4>{}, integral_constant<int, 2>{});
     }
-    else if (S.rpl == 2)
-      go(integral_constant<int, 2>{}, integral_constant<int, 2>{});
+    else if (S.rpl == 4)
+      go(integral_constant<int, 4>{}, integral_constant<int, 4>{});
     else
-      go(integral_constant<int, 2>{}, integral_constant<int, 1>{});
+      go(integral_constant<int, 4>{}, integral_constant<int, 1>{});
   }
   else
   {

@@ -650,7 +650,7 @@ void scatter_values(spmvx_matrix_s* A, const RowPatterns& R, const SliceLayout& 
 {
   const Pattern& P = *A->pat;
   const size_t es = dtype_size(A->dtype);
-  const int vw = static_cast<int>(8 / es);
+  const int vw = sliced_vw_of(static_cast<int>(es));
   const int64_t g64 = std::min<int64_t>((P.nrows + 255) / 256, 8192);
   const dim3 grid(static_cast<unsigned>(g64 > 0 ? g64 : 1)), block(256);
 #define SPMVX_SCATTER(T)                                                                    \
@@ -719,7 +719,7 @@ SlicedView sliced_view(spmvx_matrix_s* A)
 {
   SPMVX_REQUIRE(!is_complex(A->dtype), "the sliced layout is for real values");
   Pattern& P = *A->pat;
-  const int vw = static_cast<int>(8 / dtype_size(A->dtype));
+  const int vw = sliced_vw_of(static_cast<int>(dtype_size(A->dtype)));
   const RowPatterns* R = P.row_patterns();
   SPMVX_REQUIRE(R != nullptr, P.rpat_error);
   const SliceLayout& L = P.slice_layout(vw);
@@ -739,8 +739,8 @@ SlicedView sliced_view(spmvx_matrix_s* A)
     return (v == 1 || v == 2 || v == 4) ? v : 0;
   }();
   // measured on the 27- and 7-point stencils: FP64 one slice per wavefront,
-  // FP32 four, BF16 two
-  const int bs = forced ? forced : (A->dtype == SPMVX_F64 ? 1 : (A->dtype == SPMVX_F32 ? 4 : 2));
+  // FP32 and BF16 two
+  const int bs = forced ? forced : (A->dtype == SPMVX_F64 ? 1 : 2);
   return SlicedView{R->pid.as<uint16_t>(), L.sdesc.as<int2>(), L.nslices, P.nrows, P.ncols,
                     R->dict.as<int32_t>(),  R->dict_n,          R->wmax,   bs};
 }
@@ -897,7 +897,7 @@ int spmvx_matrix_set_values(spmvx_matrix A, const double* values_dev, spmvx_stre
       check_launch("convert_values");
       if (A->svalues_ready)
       {
-        const int vw = static_cast<int>(8 / dtype_size(A->dtype));
+        const int vw = sliced_vw_of(static_cast<int>(dtype_size(A->dtype)));
         scatter_values(A, *A->pat->row_patterns(), A->pat->slice_layout(vw), as_stream(stream));
       }
       if (A->tpat) // the transposed copy keeps its pattern: values re-gathered
@@ -955,7 +955,7 @@ int spmvx_matrix_get_kernel(spmvx_matrix A, int32_t* variant, char* name, int32_
         // launcher picks them from the padded pattern width
         const SlicedView sv = sliced_view(A);
         const int vb = static_cast<int>(dtype_size(A->dtype));
-        const int vw = 8 / vb;
+        const int vw = sliced_vw_of(vb);
         const int wpad = (sv.wmax + vw - 1) / vw * vw;
         d = "sliced_spmv (row-pattern slices of 64 rows, " + std::to_string(sv.rpl)
             + " per wavefront, " + std::to_string(sliced_unr(vb, sv.rpl, wpad) * vw)
@@ -990,7 +990,7 @@ int spmvx_matrix_sliced_info(spmvx_matrix A, int32_t* npatterns, int32_t* wmax, 
     Pattern& P = *A->pat;
     const RowPatterns* R = P.row_patterns();
     SPMVX_REQUIRE(R != nullptr, P.rpat_error);
-    const SliceLayout& L = P.slice_layout(static_cast<int>(8 / dtype_size(A->dtype)));
+    const SliceLayout& L = P.slice_layout(sliced_vw_of(static_cast<int>(dtype_size(A->dtype))));
     if (npatterns) *npatterns = R->npat;
     if (wmax) *wmax = R->wmax;
     if (slots) *slots = L.slots;
