@@ -71,12 +71,30 @@ __global__ void cg_state_init(CgState<Acc>* st, int32_t kmax, Acc rtol2)
 }
 
 constexpr int kGraphChunk = 16; // iterations per captured graph
+constexpr int kTailChunk = 4;   // iterations per graph near convergence
+
+// Whether the iterations still needed, extrapolated from the residual
+// decrease between two polls (k1, rho1) and the latest state, end within
+// the replay in flight plus one more full chunk: then the next replays use
+// the short graph, so that fewer no-op iterations follow convergence.
+template <typename Acc>
+bool near_convergence(const CgState<Acc>& o, int32_t k1, double rho1, int32_t in_flight)
+{
+  const double rho = static_cast<double>(o.rnorm_old);
+  if (k1 < 0 || o.k <= k1 || !(rho > 0.0) || !(rho < rho1))
+    return false;
+  const double rate = std::log(rho / rho1) / static_cast<double>(o.k - k1); // < 0
+  const double need = std::log(static_cast<double>(o.rtol2) * static_cast<double>(o.rnorm0) / rho);
+  return need / rate <= static_cast<double>(in_flight + kGraphChunk);
+}
 
 // Host driver of one CG solve.  x has ncols entries; b has nrows.
-// A chunk of kGraphChunk iterations is captured once (per matrix, x, history
-// buffer and ghost map) and replayed; the kernels are gated by the device
-// state, so replays past convergence are no-ops.  Two replays are kept in
-// flight while the host polls the stop flag of the older one.
+// Chunks of kGraphChunk and kTailChunk iterations are captured once (per
+// matrix, x, history buffer and ghost map) and replayed; the kernels are
+// gated by the device state, so replays past convergence are no-ops.  Two
+// replays are kept in flight while the host polls the state of the older
+// one, switching to the short chunk when the residual decrease so far says
+// that convergence is near.
 template <typename V, typename X, typename Acc>
 int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t vec_dtype,
            const X* b, X* x, int32_t kmax, Acc rtol, Acc* hist_dev, hipStream_t s)
@@ -100,7 +118,6 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
     SPMVX_HIP(hipEventCreateWithFlags(&w.ev_in, hipEventDisableTiming));
     SPMVX_HIP(hipEventCreateWithFlags(&w.ev_poll[0], hipEventDisableTiming));
     SPMVX_HIP(hipEventCreateWithFlags(&w.ev_poll[1], hipEventDisableTiming));
-    w.poll.ensure(2 * sizeof(int32_t));
   }
   hipStream_t q = w.stream;
   if (col_map)
@@ -133,23 +150,34 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
   const void* halo = col_map ? col_map->d_halo.ptr : nullptr;
   const int32_t halo_dt = col_map ? col_map->halo_dtype : -2;
   const uint64_t map_id = col_map ? col_map->id : 0;
-  if (!w.exec || w.g_x != x || w.g_hist != hist_dev || w.g_map != map_id || w.g_halo != halo
-      || w.g_halo_dtype != halo_dt || w.g_variant != A->variant
-      || w.g_long_rows != A->long_rows || w.g_mem != w.mem.ptr
-      || w.g_values != vals)
+  if (!w.exec || !w.exec_tail || w.g_x != x || w.g_hist != hist_dev || w.g_map != map_id
+      || w.g_halo != halo || w.g_halo_dtype != halo_dt || w.g_variant != A->variant
+      || w.g_long_rows != A->long_rows || w.g_mem != w.mem.ptr || w.g_values != vals)
   {
-    if (w.exec)
-      SPMVX_HIP(hipGraphExecDestroy(w.exec));
-    if (w.graph)
-      SPMVX_HIP(hipGraphDestroy(w.graph));
-    w.exec = nullptr;
-    w.graph = nullptr;
+    for (hipGraphExec_t* e : {&w.exec, &w.exec_tail})
+    {
+      if (*e)
+        SPMVX_HIP(hipGraphExecDestroy(*e));
+      *e = nullptr;
+    }
+    for (hipGraph_t* g : {&w.graph, &w.graph_tail})
+    {
+      if (*g)
+        SPMVX_HIP(hipGraphDestroy(*g));
+      *g = nullptr;
+    }
     w.graph = capture_graph(q, [&] {
       for (int j = 0; j < kGraphChunk; ++j)
         iteration(q);
     });
     check_launch("cg iteration capture");
     SPMVX_HIP(hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0));
+    w.graph_tail = capture_graph(q, [&] {
+      for (int j = 0; j < kTailChunk; ++j)
+        iteration(q);
+    });
+    check_launch("cg iteration capture");
+    SPMVX_HIP(hipGraphInstantiate(&w.exec_tail, w.graph_tail, nullptr, nullptr, 0));
     w.g_x = x;
     w.g_hist = hist_dev;
     w.g_map = map_id;
@@ -168,22 +196,30 @@ int run_cg(spmvx_matrix_s* A, CgWorkspace& w, spmvx_l2gmap_s* col_map, int32_t v
   cg_init<X, Acc><<<g0, kVecBS, 0, q>>>(n, ncols, b, x, r, p, part_0);
   cg_init_finalize<Acc><<<1, kFinBS, 0, q>>>(part_0, g0, st, hist_dev);
   check_launch("cg_init");
-  auto* poll = static_cast<int32_t*>(w.poll.ptr);
+  auto* poll = static_cast<CgState<Acc>*>(w.poll.ensure(2 * sizeof(CgState<double>)));
   int32_t launched = 0;
   int slot = 0, inflight = 0;
-  bool stopped = kmax <= 0;
+  int32_t chunk[2] = {0, 0};
+  bool stopped = kmax <= 0, tail = false;
+  int32_t k1 = -1; // iteration and residual of the previous poll
+  double rho1 = 0.0;
   while (!stopped && launched < kmax)
   {
-    SPMVX_HIP(hipGraphLaunch(w.exec, q));
-    launched += kGraphChunk;
-    SPMVX_HIP(hipMemcpyAsync(&poll[slot], &st->stopped, sizeof(int32_t), hipMemcpyDeviceToHost, q));
+    chunk[slot] = tail ? kTailChunk : kGraphChunk;
+    SPMVX_HIP(hipGraphLaunch(tail ? w.exec_tail : w.exec, q));
+    launched += chunk[slot];
+    SPMVX_HIP(hipMemcpyAsync(&poll[slot], st, sizeof(CgState<Acc>), hipMemcpyDeviceToHost, q));
     SPMVX_HIP(hipEventRecord(w.ev_poll[slot], q));
     slot ^= 1;
     if (++inflight == 2)
     {
       SPMVX_HIP(hipEventSynchronize(w.ev_poll[slot])); // the older replay
-      stopped = poll[slot] != 0;
+      const CgState<Acc> o = poll[slot];
+      stopped = o.stopped != 0;
       inflight = 1;
+      tail = near_convergence(o, k1, rho1, chunk[slot ^ 1]);
+      k1 = o.k;
+      rho1 = static_cast<double>(o.rnorm_old);
     }
   }
   CgState<Acc> fin{};

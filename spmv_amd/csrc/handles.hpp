@@ -247,10 +247,12 @@ struct CgWorkspace
   CgWorkspace& operator=(const CgWorkspace&) = delete;
   ~CgWorkspace()
   {
-    if (exec)
-      (void)hipGraphExecDestroy(exec);
-    if (graph)
-      (void)hipGraphDestroy(graph);
+    for (hipGraphExec_t e : {exec, exec_tail})
+      if (e)
+        (void)hipGraphExecDestroy(e);
+    for (hipGraph_t g : {graph, graph_tail})
+      if (g)
+        (void)hipGraphDestroy(g);
     for (hipEvent_t e : {ev_in, ev_poll[0], ev_poll[1]})
       if (e)
         (void)hipEventDestroy(e);
@@ -261,9 +263,11 @@ struct CgWorkspace
   hipEvent_t ev_in = nullptr;
   hipEvent_t ev_poll[2] = {nullptr, nullptr};
   PinnedBuf poll;
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr; // kGraphChunk iterations
   hipGraphExec_t exec = nullptr;
-  const void* g_x = nullptr;   // arguments the graph was captured with
+  hipGraph_t graph_tail = nullptr; // kTailChunk iterations (near convergence)
+  hipGraphExec_t exec_tail = nullptr;
+  const void* g_x = nullptr;   // arguments the graphs were captured with
   const void* g_hist = nullptr;
   uint64_t g_map = 0; // id of the column map (0: none)
   const void* g_halo = nullptr;
