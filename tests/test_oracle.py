@@ -305,6 +305,33 @@ def test_cg_f32():
     assert np.linalg.norm(x - xd) <= 1e-4 * np.linalg.norm(xd)
 
 
+def test_cg_f32_restarts_meet_true_residual():
+    """or_cg_f32_restart (the batched solver's checker): without restarts it
+    is the plain FP32 CG (same iterate, count and history); on random SPD
+    systems where the plain FP32 CG stops by its recursive residual with a
+    true residual ||b - A x|| / ||b|| above rtol (computed here in double
+    from the FP32 values), the restarted solve meets rtol, and it restarts
+    from x: a few iterations more, not a second solve from zero."""
+    from spmv_amd import problems as P
+    rtol = 1e-6
+    for n, seed in ((3000, 1002), (3000, 1006), (8000, 1000), (8000, 1005)):
+        A = P.random_spd(n, seed)
+        v32 = A.val.astype(np.float32)
+        oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, v32.astype(np.float64))
+        b = O.uniform(seed, n, -1, 1).astype(np.float32)
+
+        def true_rel(x):
+            r = b.astype(np.float64) - oc.to_scipy() @ x.astype(np.float64)
+            return np.linalg.norm(r) / np.linalg.norm(b.astype(np.float64))
+        x0, k0, h0 = O.cg_f32(oc, v32, b, 5000, rtol)
+        xz, kz, hz = O.cg_f32(oc, v32, b, 5000, rtol, restarts=0)
+        assert kz == k0 and np.array_equal(xz, x0) and np.array_equal(hz, h0)
+        assert true_rel(x0) > rtol
+        x1, k1, _ = O.cg_f32(oc, v32, b, 5000, rtol, restarts=4)
+        assert true_rel(x1) <= rtol
+        assert k0 < k1 <= k0 + k0 // 2
+
+
 # ---------------------------------------------------------------- adjoint
 def test_adjoint_dense_closed_form():
     n = 4
