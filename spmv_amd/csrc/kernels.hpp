@@ -632,14 +632,14 @@ constexpr int kSlice = 64;
 constexpr int kSlicedVariant = SPMVX_SLICED_VARIANT;  // kernel variant id of this layout
 static_assert(kSlicedVariant == kNumVariants, "the sliced layout follows the CSR variants");
 
-// Entries per lane per value load (VW) for values of vbytes bytes: one FP64
-// or FP32 value (no padding: the slice width stays the longest row), four
-// BF16 values (an 8-byte load).  Measured on the 7- and 27-point stencils:
-// FP32 with 4-byte loads and no padding ran 12-18% faster than with 8-byte
-// loads of value pairs (7 entries padded to 8).
-__host__ __device__ constexpr int sliced_vw_of(int vbytes)
+// Entries per lane per value load (VW) for values of vbytes bytes: one for
+// every value type (no padding: the slice width stays the longest row).
+// Measured on the 7- and 27-point stencils: FP32 with 4-byte loads ran
+// 12-18% faster than with 8-byte loads of value pairs (7 entries padded to
+// 8), BF16 with 2-byte loads 4-6% faster than with 8-byte loads of four.
+__host__ __device__ constexpr int sliced_vw_of(int /* vbytes */)
 {
-  return vbytes == 2 ? 4 : 1;
+  return 1;
 }
 template <typename V>
 constexpr int sliced_vw()
@@ -838,8 +838,8 @@ inline int sliced_unr(int vbytes, int rpl, int wpad)
     return 4;
   }
   if (rpl == 2)
-    return sliced_step(wpad, {4, 8, 12}) / 4;
-  return 2;
+    return sliced_step(wpad, {4, 7});
+  return 4;
 }
 
 template <typename V, typename X, typename Acc, int MODE>
@@ -905,17 +905,15 @@ int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X*
   {
     if (S.rpl == 2)
     {
-      if (u == 3)
-        go(integral_constant<int, 3>{}, integral_constant<int, 2>{});
-      else if (u == 2)
-        go(integral_constant<int, 2>{}, integral_constant<int, 2>{});
+      if (u == 7)
+        go(integral_constant<int, 7>{}, integral_constant<int, 2>{});
       else
-        go(integral_constant<int, 1>{}, integral_constant<int, 2>{});
+        go(integral_constant<int, 4>{}, integral_constant<int, 2>{});
     }
     else if (S.rpl == 4)
-      go(integral_constant<int, 2>{}, integral_constant<int, 4>{});
+      go(integral_constant<int, 4>{}, integral_constant<int, 4>{});
     else
-      go(integral_constant<int, 2>{}, integral_constant<int, 1>{});
+      go(integral_constant<int, 4>{}, integral_constant<int, 1>{});
   }
   return nblk;
 }

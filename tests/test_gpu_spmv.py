@@ -547,10 +547,10 @@ def band_matrix(n, lo, hi, seed):
 
 SLICED_WIDTHS = [(1, 1), (2, 2), (4, 4), (7, 7), (8, 7), (8, 8), (14, 14), (20, 20)]
 # slots per step the launcher picks for each width above (fewest idle slots,
-# then fewest steps, among the compiled steps): FP64 4/7/8/10/14, FP32
-# 4/7/8/14 (one value per load, no padding), BF16 4/8/12 (four per load)
+# then fewest steps, among the compiled steps; one value per load, no
+# padding): FP64 4/7/8/10/14, FP32 4/7/8/14, BF16 4/7
 SLICED_STEPS = {"f64": [4, 7, 10, 8, 8, 10, 10, 14], "f32": [4, 7, 4, 8, 8, 4, 8, 14],
-                "bf16": [4, 8, 12, 8, 8, 4, 8, 4]}
+                "bf16": [4, 7, 4, 4, 4, 4, 4, 7]}
 _steps_seen = {dt: set() for dt in SLICED_STEPS}
 
 
@@ -590,7 +590,7 @@ def test_sliced_layout_every_step_used():
     """The width cases above cover every compiled step (runs after them)."""
     if not all(_steps_seen.values()):
         pytest.skip("runs after the width cases")
-    assert _steps_seen == {"f64": {4, 7, 8, 10, 14}, "f32": {4, 7, 8, 14}, "bf16": {4, 8, 12}}
+    assert _steps_seen == {"f64": {4, 7, 8, 10, 14}, "f32": {4, 7, 8, 14}, "bf16": {4, 7}}
 
 
 def test_sliced_layout_info():
@@ -604,7 +604,7 @@ def test_sliced_layout_info():
     w = [lens[64 * i:64 * (i + 1)].max() for i in range(nsl)]
     assert info["slots"] == 64 * sum(w)  # FP64: one entry per lane and load
     assert M.like("f32").sliced_info()["slots"] == 64 * sum(w)  # FP32 too
-    assert M.like("bf16").sliced_info()["slots"] == 64 * sum(-(-x // 4) * 4 for x in w)
+    assert M.like("bf16").sliced_info()["slots"] == 64 * sum(w)  # and BF16
 
 
 @pytest.mark.parametrize("name,A,reason", [
