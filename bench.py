@@ -482,7 +482,8 @@ def main():
     step_ms_ev = e_all0.elapsed_time(e_all1) / args.steps
     ms_per_step = 1e3 * wall / args.steps
     value = B / (ms_per_step * 1e-3) / 1e9
-    spmv_ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    per_step_ms = sorted(a.elapsed_time(b) for a, b in ev)
+    spmv_ms = sum(per_step_ms) / args.steps
     achieved = B / (spmv_ms * 1e-3) / 1e9
     # the same kernel back to back (no update between), for reference
     spmv_alone_ms = timed_loop(torch, lambda: M.mult(x, out=y), max(args.steps, 20))
@@ -603,6 +604,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel_ms": round(spmv_ms, 5),
                      "kernel_ms_alone": round(spmv_alone_ms, 5),
+                     "kernel_ms_median": round(per_step_ms[len(per_step_ms) // 2], 5),
+                     "kernel_ms_min": round(per_step_ms[0], 5),
                      "kernel": M.kernel_name(),
                      "hbm_GBps_measured": (round(traffic / (spmv_ms * 1e-3) / 1e9, 1)
                                            if traffic else None),

@@ -47,7 +47,40 @@ public:
 
   spmvx_matrix handle() const { return _h; }
 
+  /// Matrix::create_matrix (Matrix.h:55-65, Matrix.cpp:213-438): the local
+  /// block `mat` (nrows_local + row_ghosts.size() rows, ncols_local +
+  /// col_ghosts.size() columns, ghost columns numbered by their position in
+  /// col_ghosts) becomes a matrix with column ghosts only: ghost columns
+  /// renumbered in ascending global order (duplicates merged), entries of
+  /// a row sorted by column with duplicates summed (setFromTriplets).  The
+  /// reference sends ghost rows to their owners; one process drives one rank
+  /// here, so row_ghosts must be empty (runtime_error otherwise).  Without a
+  /// Decomposition the rank has no neighbours and col_ghosts must be empty
+  /// too; with one, its ghost columns are owned by the other (virtual)
+  /// ranks it describes.
+  static Matrix<T> create_matrix(MPI_Comm comm, const CSR<T>& mat, std::int64_t nrows_local,
+                                 std::int64_t ncols_local, std::vector<std::int64_t> row_ghosts,
+                                 std::vector<std::int64_t> col_ghosts);
+  static Matrix<T> create_matrix(MPI_Comm comm, const CSR<T>& mat, std::int64_t nrows_local,
+                                 std::int64_t ncols_local, std::vector<std::int64_t> row_ghosts,
+                                 std::vector<std::int64_t> col_ghosts,
+                                 const Decomposition& decomp);
+#ifdef SPMV_HAVE_EIGEN
+  static Matrix<T> create_matrix(MPI_Comm comm, const Eigen::SparseMatrix<T, Eigen::RowMajor> mat,
+                                 std::int64_t nrows_local, std::int64_t ncols_local,
+                                 std::vector<std::int64_t> row_ghosts,
+                                 std::vector<std::int64_t> col_ghosts)
+  {
+    return create_matrix(comm, CSR<T>(mat), nrows_local, ncols_local, std::move(row_ghosts),
+                         std::move(col_ghosts));
+  }
+#endif
+
 private:
+  static Matrix<T> create_local(MPI_Comm comm, const CSR<T>& mat, std::int64_t nrows_local,
+                                std::int64_t ncols_local, const std::vector<std::int64_t>& row_ghosts,
+                                const std::vector<std::int64_t>& col_ghosts,
+                                const Decomposition* decomp);
   spmvx_matrix _h = nullptr;
   std::int64_t _rows = 0, _cols = 0, _nnz = 0;
   std::shared_ptr<L2GMap> _col_map, _row_map;

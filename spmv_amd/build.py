@@ -43,7 +43,8 @@ def _compile(src):
 
 def build_cpp(verbose=False):
     """The C++ API (libspmv.so: spmv::L2GMap, spmv::Matrix, spmv::cg over the
-    C ABI) and the demo drivers (bin/matvecmain, bin/cgmain, bin/restrictmain).  Needs an MPI
+    C ABI) and the demo drivers (bin/matvecmain, bin/cgmain, bin/restrictmain, and
+    bin/createmain, a driver of Matrix::create_matrix for the tests).  Needs an MPI
     installation (the API keeps the reference's MPI_Comm arguments)."""
     mpi_inc = os.path.join(MPI_HOME, "include")
     mpi_lib = os.path.join(MPI_HOME, "lib")
@@ -70,7 +71,7 @@ def build_cpp(verbose=False):
                            "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/_mpi"])
     os.replace(tmp, CXXLIB)
     os.makedirs(BIN, exist_ok=True)
-    for demo in ("matvecmain", "cgmain", "restrictmain"):
+    for demo in ("matvecmain", "cgmain", "restrictmain", "createmain"):
         out = os.path.join(BIN, demo)
         subprocess.check_call([HIPCC, "-O2", "-std=c++17", "-x", "c++", *inc,
                                os.path.join(HERE, "demos", demo + ".cpp"), "-o", out + ".tmp",

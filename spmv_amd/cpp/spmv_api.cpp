@@ -3,11 +3,14 @@
 // message, which for the reference's checks is the reference's text.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <complex>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "spmv/spmv.h"
 
@@ -354,6 +357,97 @@ CSR<T> Matrix<T>::mat() const
               "hipMemcpy");
   }
   return A;
+}
+
+template <typename T>
+Matrix<T> Matrix<T>::create_local(MPI_Comm comm, const CSR<T>& mat, std::int64_t nrows_local,
+                                  std::int64_t ncols_local,
+                                  const std::vector<std::int64_t>& row_ghosts,
+                                  const std::vector<std::int64_t>& col_ghosts,
+                                  const Decomposition* decomp)
+{
+  if (!row_ghosts.empty())
+    throw std::runtime_error("spmv::Matrix::create_matrix: ghost rows go to the ranks that own "
+                             "them; one process drives one rank, so row_ghosts must be empty");
+  if (nrows_local < 0 || ncols_local < 0 || mat.rows != nrows_local
+      || mat.cols != ncols_local + static_cast<std::int64_t>(col_ghosts.size())
+      || static_cast<std::int64_t>(mat.outer.size()) != mat.rows + 1
+      || mat.inner.size() != static_cast<std::size_t>(mat.nonZeros())
+      || mat.values.size() != static_cast<std::size_t>(mat.nonZeros()))
+    throw std::runtime_error("spmv::Matrix::create_matrix: matrix sizes differ from the maps");
+  // ghost columns in ascending global order, each once (Matrix.cpp:342-367)
+  std::vector<std::int64_t> ghosts(col_ghosts);
+  std::sort(ghosts.begin(), ghosts.end());
+  ghosts.erase(std::unique(ghosts.begin(), ghosts.end()), ghosts.end());
+  auto local_col = [&](std::int32_t c) -> std::int32_t {
+    if (c < 0 || c >= mat.cols)
+      throw std::runtime_error("spmv::Matrix::create_matrix: column index out of range");
+    if (c < ncols_local)
+      return c;
+    const std::int64_t g = col_ghosts[static_cast<std::size_t>(c - ncols_local)];
+    return static_cast<std::int32_t>(
+        ncols_local + (std::lower_bound(ghosts.begin(), ghosts.end(), g) - ghosts.begin()));
+  };
+  // rows: entries sorted by column, duplicates summed in stored order
+  // (Eigen's setFromTriplets, Matrix.cpp:370-384,421-424)
+  CSR<T> B;
+  B.rows = nrows_local;
+  B.cols = ncols_local + static_cast<std::int64_t>(ghosts.size());
+  B.outer.assign(static_cast<std::size_t>(nrows_local) + 1, 0);
+  std::vector<std::pair<std::int32_t, T>> row;
+  for (std::int64_t r = 0; r < nrows_local; ++r)
+  {
+    row.clear();
+    for (std::int32_t k = mat.outer[static_cast<std::size_t>(r)];
+         k < mat.outer[static_cast<std::size_t>(r) + 1]; ++k)
+      row.emplace_back(local_col(mat.inner[static_cast<std::size_t>(k)]),
+                       mat.values[static_cast<std::size_t>(k)]);
+    std::stable_sort(row.begin(), row.end(),
+                     [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (std::size_t i = 0; i < row.size(); ++i)
+    {
+      if (i > 0 && row[i].first == row[i - 1].first)
+        B.values.back() += row[i].second;
+      else
+      {
+        B.inner.push_back(row[i].first);
+        B.values.push_back(row[i].second);
+      }
+    }
+    B.outer[static_cast<std::size_t>(r) + 1] = static_cast<std::int32_t>(B.inner.size());
+  }
+  std::shared_ptr<L2GMap> col_map, row_map;
+  if (decomp)
+  {
+    Decomposition rows_only = *decomp;
+    for (auto& g : rows_only.remote_ghosts)
+      g.clear();
+    col_map = std::make_shared<L2GMap>(comm, ncols_local, ghosts, *decomp);
+    row_map = std::make_shared<L2GMap>(comm, nrows_local, std::vector<std::int64_t>(), rows_only);
+  }
+  else
+  {
+    col_map = std::make_shared<L2GMap>(comm, ncols_local, ghosts);
+    row_map = std::make_shared<L2GMap>(comm, nrows_local, std::vector<std::int64_t>());
+  }
+  return Matrix<T>(B, col_map, row_map);
+}
+
+template <typename T>
+Matrix<T> Matrix<T>::create_matrix(MPI_Comm comm, const CSR<T>& mat, std::int64_t nrows_local,
+                                   std::int64_t ncols_local, std::vector<std::int64_t> row_ghosts,
+                                   std::vector<std::int64_t> col_ghosts)
+{
+  return create_local(comm, mat, nrows_local, ncols_local, row_ghosts, col_ghosts, nullptr);
+}
+
+template <typename T>
+Matrix<T> Matrix<T>::create_matrix(MPI_Comm comm, const CSR<T>& mat, std::int64_t nrows_local,
+                                   std::int64_t ncols_local, std::vector<std::int64_t> row_ghosts,
+                                   std::vector<std::int64_t> col_ghosts,
+                                   const Decomposition& decomp)
+{
+  return create_local(comm, mat, nrows_local, ncols_local, row_ghosts, col_ghosts, &decomp);
 }
 
 template class Matrix<double>;

@@ -83,3 +83,40 @@ def test_restrictmain_petsc(tmp_path, mode):
     assert field(out, "mat.equal") == 1
     assert abs(field(out, "pnorm") - pn) <= 1e-12 * pn
     assert abs(field(out, "qnorm") - qn) <= 1e-12 * qn
+
+
+def test_create_matrix_one_rank_of_two():
+    """Matrix::create_matrix (Matrix.cpp:213-438) for rank 0 of a two-rank
+    decomposition: ghost columns renumbered in ascending global order (a
+    repeated ghost merged), entries of each row sorted by column with
+    repeated columns summed in stored order; the column map carries the new
+    ghosts; a call with ghost rows is refused (one process drives one rank)."""
+    out = run("createmain")
+    rows_in = [[(1, 2.0), (0, 4.0), (6, -1.0), (4, -0.5)], [(1, 3.0), (1, 0.25)],
+               [(5, 7.0), (2, -2.0), (3, 1.5)], [(3, 6.0), (2, -3.0)]]
+    col_ghosts = [7, 5, 7]
+    ghosts = sorted(set(col_ghosts))
+    glob = lambda c: c if c < 4 else col_ghosts[c - 4]
+    new = lambda g: g if g < 4 else 4 + ghosts.index(g)
+    outer, inner, values = [0], [], []
+    for row in rows_in:
+        acc = {}
+        for c, v in row:  # stored order: duplicates summed in that order
+            k = new(glob(c))
+            acc[k] = acc[k] + v if k in acc else v
+        for k in sorted(acc):
+            inner.append(k)
+            values.append(acc[k])
+        outer.append(len(inner))
+
+    def ints(name):
+        return [int(t) for t in re.search(rf"^{name} =(.*)$", out, re.M).group(1).split()]
+
+    def floats(name):
+        return [float(t) for t in re.search(rf"^{name} =(.*)$", out, re.M).group(1).split()]
+    assert field(out, "rows") == 4 and field(out, "cols") == 4 + len(ghosts)
+    assert ints("outer") == outer and ints("inner") == inner and floats("values") == values
+    assert ints("ghosts") == ghosts
+    y = [sum(values[k] * (inner[k] + 1) for k in range(outer[r], outer[r + 1])) for r in range(4)]
+    assert floats("y") == y
+    assert "row_ghosts must be empty" in re.search(r"^error = (.*)$", out, re.M).group(1)
