@@ -235,14 +235,18 @@ constexpr unsigned kSpinLimit = 1u << 22;
 // Barrier among the W workgroups of one team (all of them are running: a
 // workgroup joins a team only by taking one of its slots from the queue).
 // Arrivals are counted on a per-system counter; the t-th barrier of a solve
-// completes at W * t arrivals.  Agent-scope fences make the vector slices
-// and partial sums written before the barrier visible to every member.
+// completes at W * t arrivals.  An agent-scope release before the arrival
+// and an acquire after the wait make the vector slices and partial sums
+// written before the barrier visible to every member (members may run on
+// different XCDs, whose L2s are not coherent with each other: the release
+// writes this XCD's L2 back, the acquire invalidates it; a full fence on
+// both sides would do each twice).
 __device__ __forceinline__ void team_barrier(unsigned* bar, unsigned target, int* err)
 {
   __syncthreads();
   if (threadIdx.x == 0)
   {
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     atomicAdd(bar, 1u);
     unsigned spins = 0;
     while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
@@ -254,7 +258,7 @@ __device__ __forceinline__ void team_barrier(unsigned* bar, unsigned target, int
         break;
       }
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }
