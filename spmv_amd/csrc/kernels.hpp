@@ -921,6 +921,43 @@ int64_t launch_sliced(const SlicedView& S, hipStream_t s, const V* val, const X*
 // ---------------------------------------------------------------- CG vectors
 constexpr int kVecBS = 256;
 
+// 16 bytes of a real vector (4 floats or 2 doubles) for the streaming vector
+// kernels: with one float per lane and load, a grid-stride pass keeps too few
+// bytes in flight per CU to reach HBM bandwidth (measured on 16.8M-row CG:
+// the float updates run 6-7% faster with 16-byte passes).  Doubles keep one
+// element per lane: the 16-byte passes made FP64 CG about 1.5% slower.
+template <typename X>
+constexpr bool kVec16 = sizeof(X) == 4;
+template <typename X>
+struct Pack16
+{
+  static constexpr int N = 16 / static_cast<int>(sizeof(X));
+  X v[N];
+};
+
+template <typename X>
+__device__ __forceinline__ Pack16<X> ld16(const X* p)
+{
+  Pack16<X> r;
+  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  __builtin_memcpy(r.v, &q, 16);
+  return r;
+}
+
+template <typename X>
+__device__ __forceinline__ void st16(X* p, const Pack16<X>& r)
+{
+  uint4 q;
+  __builtin_memcpy(&q, r.v, 16);
+  *reinterpret_cast<uint4*>(p) = q;
+}
+
+__device__ __forceinline__ bool aligned16(const void* a, const void* b, const void* c = nullptr)
+{
+  return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)
+           | reinterpret_cast<uintptr_t>(c)) & 15u) == 0;
+}
+
 // x = 0, r = b, p[0:n) = b, p[n:ncols) = 0, partial ||b||^2 (cg.cpp:31-44)
 template <typename X, typename Acc>
 __global__ __launch_bounds__(kVecBS) void cg_init(int64_t n, int64_t ncols,
@@ -1012,8 +1049,43 @@ __global__ __launch_bounds__(kVecBS) void cg_update_r(int64_t n, X* __restrict__
   __shared__ Acc red[16];
   const X alpha = static_cast<X>(st->alpha);
   Acc part = Acc(0);
+  constexpr int N = Pack16<X>::N;
+  const int64_t gid = blockIdx.x * static_cast<int64_t>(kVecBS) + threadIdx.x;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kVecBS;
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(kVecBS) + threadIdx.x; i < n; i += stride)
+  int64_t done = 0; // elements covered by the 16-byte passes
+  if (kVec16<X> && aligned16(r, y))
+  {
+    // two 16-byte packs per thread and step, both loaded before either is
+    // used (the second re-reads the first when past the end; stored only
+    // when in range)
+    const int64_t npk = n / N;
+    for (int64_t q = gid; q < npk; q += 2 * stride)
+    {
+      const bool two = q + stride < npk;
+      const int64_t q2 = two ? q + stride : q;
+      Pack16<X> ra = ld16(r + q * N), ya = ld16(y + q * N);
+      Pack16<X> rb = ld16(r + q2 * N), yb = ld16(y + q2 * N);
+#pragma unroll
+      for (int e = 0; e < N; ++e)
+      {
+        ra.v[e] = ra.v[e] - alpha * ya.v[e];
+        part += static_cast<Acc>(ra.v[e]) * static_cast<Acc>(ra.v[e]);
+      }
+      st16(r + q * N, ra);
+      if (two)
+      {
+#pragma unroll
+        for (int e = 0; e < N; ++e)
+        {
+          rb.v[e] = rb.v[e] - alpha * yb.v[e];
+          part += static_cast<Acc>(rb.v[e]) * static_cast<Acc>(rb.v[e]);
+        }
+        st16(r + q2 * N, rb);
+      }
+    }
+    done = npk * N;
+  }
+  for (int64_t i = done + gid; i < n; i += stride)
   {
     const X ri = r[i] - alpha * y[i];
     r[i] = ri;
@@ -1061,8 +1133,30 @@ __global__ __launch_bounds__(kVecBS) void cg_update_xp(int64_t n, X* __restrict_
     return;
   const X alpha = static_cast<X>(st->alpha);
   const X beta = static_cast<X>(st->beta);
+  constexpr int N = Pack16<X>::N;
+  const int64_t gid = blockIdx.x * static_cast<int64_t>(kVecBS) + threadIdx.x;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kVecBS;
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(kVecBS) + threadIdx.x; i < n; i += stride)
+  int64_t done = 0; // elements covered by the 16-byte passes
+  if (kVec16<X> && aligned16(x, p, r))
+  {
+    const int64_t npk = n / N;
+    for (int64_t q = gid; q < npk; q += stride)
+    {
+      Pack16<X> xa = ld16(x + q * N), pa = ld16(p + q * N);
+      const Pack16<X> ra = ld16(r + q * N);
+#pragma unroll
+      for (int e = 0; e < N; ++e)
+      {
+        const X pi = pa.v[e];
+        xa.v[e] = xa.v[e] + alpha * pi;
+        pa.v[e] = pi * beta + ra.v[e];
+      }
+      st16(x + q * N, xa);
+      st16(p + q * N, pa);
+    }
+    done = npk * N;
+  }
+  for (int64_t i = done + gid; i < n; i += stride)
   {
     const X pi = p[i];
     x[i] = x[i] + alpha * pi;

@@ -170,6 +170,32 @@ def test_cg_f32_inner_solver():
     assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-4 * np.linalg.norm(xo)
 
 
+def test_cg_f32_ragged_length_and_unaligned_vectors():
+    """The FP32 vector updates run 16-byte passes when r, y, x and p are
+    16-byte aligned and finish the last n % 4 entries one per lane: n = 1001
+    covers the tail, and b and x one float past a 16-byte boundary cover the
+    one-per-lane fallback for x.  Both meet the oracle's FP32 CG."""
+    import ctypes
+    A = P.stencil(7, 7, 11, 13)
+    assert A.nrows % 4 == 1
+    b = P.uniform(4, A.nrows, -1, 1).astype(np.float32)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols, dtype="f32")
+    xo, ko, _ = O.cg_f32(_ocsr(A), A.val.astype(np.float32), b, 5000, 1e-5)
+    x, k = S.cg_f32(M, _dev(b, torch.float32), 5000, 1e-5)
+    assert abs(k - ko) <= 3
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-4 * np.linalg.norm(xo)
+    bb = torch.zeros(A.nrows + 1, dtype=torch.float32, device=DEV)
+    xx = torch.zeros(A.ncols + 1, dtype=torch.float32, device=DEV)
+    bb[1:] = _dev(b, torch.float32)
+    assert bb[1:].data_ptr() % 16 == 4 and xx[1:].data_ptr() % 16 == 4
+    its = ctypes.c_int32()
+    S._check(S.lib().spmvx_cg_f32(M.handle, S._dptr(bb[1:]), S._dptr(xx[1:]), 5000, 1e-5,
+                                  ctypes.byref(its), None))
+    assert abs(its.value - ko) <= 3
+    assert np.linalg.norm(xx[1:].cpu().numpy() - xo) <= 1e-4 * np.linalg.norm(xo)
+    assert xx[0].item() == 0.0
+
+
 # ---------------------------------------------------------------- sliced layout
 @pytest.mark.parametrize("name,A,rtol", [
     ("poisson7_24", P.stencil(7, 24, 24, 24), 1e-10),
