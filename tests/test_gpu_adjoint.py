@@ -119,3 +119,42 @@ def test_config_diffusion_1024_full_size():
     fd = AD.central_difference(op, th.detach(), bd.detach(), xtd, entries, eps=1e-6, rtol=1e-13)
     g = th.grad.cpu().numpy()[entries]
     assert np.max(np.abs(fd - g) / np.maximum(np.abs(g), 1e-300)) <= 1e-6
+
+
+def test_config_diffusion_1024_matches_oracle_adjoint():
+    """The full-size 1024^2 problem against the oracle's adjoint on the host
+    (the reference's CG for the forward and the adjoint solve, products and
+    axpys over the host's threads, dot products in a fixed 64-chunk order):
+    loss, grad_b = lambda and grad_theta = -(lambda_i - lambda_j)(x_i - x_j)
+    per edge.  The two CG runs round their dot products differently (8048 +
+    7469 iterations on the GPU, 8065 + 7483 on the host), so they agree to
+    1e-14 relative in x and lambda and 7e-12 in grad_theta, not bit for bit;
+    the bounds hold a margin of ~1000."""
+    import os
+    n = 1024
+    theta, b, xt = problem(n, 1)
+    op = AD.DiffusionOperator(n)
+    th = torch.from_numpy(theta).to(DEV).requires_grad_(True)
+    bd = torch.from_numpy(b).to(DEV).requires_grad_(True)
+    xtd = torch.from_numpy(xt).to(DEV)
+    x = op.solve(th, bd, rtol=1e-13)
+    loss = 0.5 * ((x - xtd) ** 2).sum()
+    loss.backward()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    A = O.gen_diffusion(n, theta)
+    xo, k1, _ = O.cg(A, b, 100000, 1e-13, threads=threads)
+    rhs = xo - xt
+    lo = 0.5 * float(rhs @ rhs)
+    lam, k2, _ = O.cg(A, rhs, 100000, 1e-13, threads=threads)
+    rows = np.repeat(np.arange(A.nrows), np.diff(A.rowptr))
+    go = O.diffusion_grad_theta(n, A, -lam[rows] * xo[A.colidx])
+    xg = x.detach().cpu().numpy()
+    lg = bd.grad.cpu().numpy()
+    gt = th.grad.cpu().numpy()
+    ex = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    el = np.linalg.norm(lg - lam) / np.linalg.norm(lam)
+    eg = np.max(np.abs(gt - go)) / np.max(np.abs(go))
+    print(f"iterations cpu {k1}+{k2}; rel diff x {ex:.2e} lambda {el:.2e} grad_theta {eg:.2e} "
+          f"loss {abs(loss.item() - lo) / lo:.2e}", flush=True)
+    assert abs(loss.item() - lo) <= 1e-12 * lo
+    assert ex <= 1e-11 and el <= 1e-11 and eg <= 1e-8
