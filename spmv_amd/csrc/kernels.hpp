@@ -519,7 +519,7 @@ struct SpmvVariant
   X(8, 8192, 2, 4, true, 1024, true)                                                         \
   X(9, 2048, 4, 1, true, 256, true)                                                          \
   X(10, 2048, 4, 2, true, 256, true)                                                         \
-  X(11, 2048, 2, 4, true, 256, true)
+  X(11, 4096, 4, 2, true, 512, true)
 constexpr int kNumVariants = 12;
 #define SPMVX_VARIANT_ENTRY(i, C, V, U, N, B, I) {C, V, U, N, B, I},
 constexpr SpmvVariant kVariants[kNumVariants] = {SPMVX_VARIANTS(SPMVX_VARIANT_ENTRY)};

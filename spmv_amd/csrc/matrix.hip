@@ -394,9 +394,12 @@ int matrix_default_variant(spmvx_matrix_s* A)
   const int base = default_variant(A->dtype);
   if (forced_variant() >= 0 || is_complex(A->dtype))
     return base;
-  // the short-index counterpart of the default when the matrix fits it (the
-  // analysis of the inspector, Matrix.cpp:43)
-  const int sv = base == 5 ? 8 : (base == 3 ? 9 : (base == 4 ? 10 : -1));
+  // the short-index kernel for the dtype when the matrix fits it (the
+  // analysis of the inspector, Matrix.cpp:43): FP64 the 16-bit counterpart
+  // of its default; FP32 and BF16 512-thread workgroups staging 4096 entries
+  // (vec 4 x unroll 2), 1-5% faster than the counterparts of their 32-bit
+  // defaults (variants 9, 10) on the 7- and 27-point stencils
+  const int sv = base == 5 ? 8 : ((base == 3 || base == 4) ? 11 : -1);
   Pattern& P = *A->pat;
   if (sv < 0 || P.nnz == 0)
     return base;
