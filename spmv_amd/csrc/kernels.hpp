@@ -518,7 +518,7 @@ struct SpmvVariant
   X(7, 8192, 4, 2, true, 1024, false)                                                        \
   X(8, 8192, 2, 4, true, 1024, true)                                                         \
   X(9, 2048, 4, 1, true, 256, true)                                                          \
-  X(10, 2048, 4, 2, true, 256, true)                                                         \
+  X(10, 4096, 8, 1, true, 512, true)                                                         \
   X(11, 4096, 4, 2, true, 512, true)
 constexpr int kNumVariants = 12;
 #define SPMVX_VARIANT_ENTRY(i, C, V, U, N, B, I) {C, V, U, N, B, I},

@@ -396,10 +396,11 @@ int matrix_default_variant(spmvx_matrix_s* A)
     return base;
   // the short-index kernel for the dtype when the matrix fits it (the
   // analysis of the inspector, Matrix.cpp:43): FP64 the 16-bit counterpart
-  // of its default; FP32 and BF16 512-thread workgroups staging 4096 entries
-  // (vec 4 x unroll 2), 1-5% faster than the counterparts of their 32-bit
-  // defaults (variants 9, 10) on the 7- and 27-point stencils
-  const int sv = base == 5 ? 8 : ((base == 3 || base == 4) ? 11 : -1);
+  // of its default; FP32 and BF16 512-thread workgroups staging 4096 entries,
+  // FP32 with vec 4 x unroll 2 (variant 11), BF16 with 16-byte value loads
+  // (vec 8, variant 10): 1-5% (FP32) and 4-7% (BF16) faster per product on
+  // the 7- and 27-point stencils than 2048-entry blocks of 256 threads
+  const int sv = base == 5 ? 8 : (base == 3 ? 11 : (base == 4 ? 10 : -1));
   Pattern& P = *A->pat;
   if (sv < 0 || P.nnz == 0)
     return base;
