@@ -935,22 +935,38 @@ struct Pack16
   X v[N];
 };
 
-template <typename X>
+// NT: a non-temporal access (streamed, not kept in the caches)
+template <bool NT = false, typename X>
 __device__ __forceinline__ Pack16<X> ld16(const X* p)
 {
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
   Pack16<X> r;
-  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  const u4 q = ld_stream<NT>(reinterpret_cast<const u4*>(p));
   __builtin_memcpy(r.v, &q, 16);
   return r;
 }
 
-template <typename X>
+template <bool NT = false, typename X>
 __device__ __forceinline__ void st16(X* p, const Pack16<X>& r)
 {
-  uint4 q;
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  u4 q;
   __builtin_memcpy(&q, r.v, 16);
-  *reinterpret_cast<uint4*>(p) = q;
+  if constexpr (NT)
+    __builtin_nontemporal_store(q, reinterpret_cast<u4*>(p));
+  else
+    *reinterpret_cast<u4*>(p) = q;
 }
+
+// Cache policy of the CG vectors.  Per iteration the product gathers p and
+// writes y, the residual update reads y and rewrites r, the last update reads
+// r and rewrites p and x: x is touched by that one kernel only and y is last
+// read by the residual update, so those accesses stream (non-temporal) and
+// leave the last-level cache to p and r.  Measured on 16.8M-row CG: x
+// streamed took FP64 CG 3-4% and FP32 refinement 5% faster; y streamed as
+// well another 3% for FP64 and nothing for FP32 (kept for doubles only).
+template <typename X>
+constexpr bool kStreamY = sizeof(X) == 8;
 
 __device__ __forceinline__ bool aligned16(const void* a, const void* b, const void* c = nullptr)
 {
@@ -1063,8 +1079,8 @@ __global__ __launch_bounds__(kVecBS) void cg_update_r(int64_t n, X* __restrict__
     {
       const bool two = q + stride < npk;
       const int64_t q2 = two ? q + stride : q;
-      Pack16<X> ra = ld16(r + q * N), ya = ld16(y + q * N);
-      Pack16<X> rb = ld16(r + q2 * N), yb = ld16(y + q2 * N);
+      Pack16<X> ra = ld16(r + q * N), ya = ld16<kStreamY<X>>(y + q * N);
+      Pack16<X> rb = ld16(r + q2 * N), yb = ld16<kStreamY<X>>(y + q2 * N);
 #pragma unroll
       for (int e = 0; e < N; ++e)
       {
@@ -1087,7 +1103,7 @@ __global__ __launch_bounds__(kVecBS) void cg_update_r(int64_t n, X* __restrict__
   }
   for (int64_t i = done + gid; i < n; i += stride)
   {
-    const X ri = r[i] - alpha * y[i];
+    const X ri = r[i] - alpha * ld_stream<kStreamY<X>>(y + i);
     r[i] = ri;
     part += static_cast<Acc>(ri) * static_cast<Acc>(ri);
   }
@@ -1142,7 +1158,7 @@ __global__ __launch_bounds__(kVecBS) void cg_update_xp(int64_t n, X* __restrict_
     const int64_t npk = n / N;
     for (int64_t q = gid; q < npk; q += stride)
     {
-      Pack16<X> xa = ld16(x + q * N), pa = ld16(p + q * N);
+      Pack16<X> xa = ld16<true>(x + q * N), pa = ld16(p + q * N);
       const Pack16<X> ra = ld16(r + q * N);
 #pragma unroll
       for (int e = 0; e < N; ++e)
@@ -1151,7 +1167,7 @@ __global__ __launch_bounds__(kVecBS) void cg_update_xp(int64_t n, X* __restrict_
         xa.v[e] = xa.v[e] + alpha * pi;
         pa.v[e] = pi * beta + ra.v[e];
       }
-      st16(x + q * N, xa);
+      st16<true>(x + q * N, xa);
       st16(p + q * N, pa);
     }
     done = npk * N;
@@ -1159,7 +1175,7 @@ __global__ __launch_bounds__(kVecBS) void cg_update_xp(int64_t n, X* __restrict_
   for (int64_t i = done + gid; i < n; i += stride)
   {
     const X pi = p[i];
-    x[i] = x[i] + alpha * pi;
+    __builtin_nontemporal_store(ld_stream<true>(x + i) + alpha * pi, x + i);
     p[i] = pi * beta + r[i];
   }
 }
