@@ -960,13 +960,15 @@ __device__ __forceinline__ void st16(X* p, const Pack16<X>& r)
 
 // Cache policy of the CG vectors.  Per iteration the product gathers p and
 // writes y, the residual update reads y and rewrites r, the last update reads
-// r and rewrites p and x: x is touched by that one kernel only and y is last
-// read by the residual update, so those accesses stream (non-temporal) and
-// leave the last-level cache to p and r.  Measured on 16.8M-row CG: x
-// streamed took FP64 CG 3-4% and FP32 refinement 5% faster; y streamed as
-// well another 3% for FP64 and nothing for FP32 (kept for doubles only).
+// r and rewrites p and x: x is touched by that one kernel only, so its
+// accesses stream (non-temporal) and leave the last-level cache to p, r and
+// y; for doubles the last reads of y (residual update) and of r (x/p update)
+// in an iteration stream as well.  Measured on 16.8M-row CG: x streamed took
+// FP64 CG 3-4% and FP32 refinement 5% faster; y streamed as well another 3%
+// for FP64 and nothing for FP32, r another 1-2% for FP64 and 1-1.5% slower
+// for FP32; p's load streamed made both slower.
 template <typename X>
-constexpr bool kStreamY = sizeof(X) == 8;
+constexpr bool kStreamLastRead = sizeof(X) == 8;
 
 __device__ __forceinline__ bool aligned16(const void* a, const void* b, const void* c = nullptr)
 {
@@ -1079,8 +1081,8 @@ __global__ __launch_bounds__(kVecBS) void cg_update_r(int64_t n, X* __restrict__
     {
       const bool two = q + stride < npk;
       const int64_t q2 = two ? q + stride : q;
-      Pack16<X> ra = ld16(r + q * N), ya = ld16<kStreamY<X>>(y + q * N);
-      Pack16<X> rb = ld16(r + q2 * N), yb = ld16<kStreamY<X>>(y + q2 * N);
+      Pack16<X> ra = ld16(r + q * N), ya = ld16<kStreamLastRead<X>>(y + q * N);
+      Pack16<X> rb = ld16(r + q2 * N), yb = ld16<kStreamLastRead<X>>(y + q2 * N);
 #pragma unroll
       for (int e = 0; e < N; ++e)
       {
@@ -1103,7 +1105,7 @@ __global__ __launch_bounds__(kVecBS) void cg_update_r(int64_t n, X* __restrict__
   }
   for (int64_t i = done + gid; i < n; i += stride)
   {
-    const X ri = r[i] - alpha * ld_stream<kStreamY<X>>(y + i);
+    const X ri = r[i] - alpha * ld_stream<kStreamLastRead<X>>(y + i);
     r[i] = ri;
     part += static_cast<Acc>(ri) * static_cast<Acc>(ri);
   }
@@ -1159,7 +1161,7 @@ __global__ __launch_bounds__(kVecBS) void cg_update_xp(int64_t n, X* __restrict_
     for (int64_t q = gid; q < npk; q += stride)
     {
       Pack16<X> xa = ld16<true>(x + q * N), pa = ld16(p + q * N);
-      const Pack16<X> ra = ld16(r + q * N);
+      const Pack16<X> ra = ld16<kStreamLastRead<X>>(r + q * N);
 #pragma unroll
       for (int e = 0; e < N; ++e)
       {
@@ -1176,7 +1178,7 @@ __global__ __launch_bounds__(kVecBS) void cg_update_xp(int64_t n, X* __restrict_
   {
     const X pi = p[i];
     __builtin_nontemporal_store(ld_stream<true>(x + i) + alpha * pi, x + i);
-    p[i] = pi * beta + r[i];
+    p[i] = pi * beta + ld_stream<kStreamLastRead<X>>(r + i);
   }
 }
 
