@@ -196,6 +196,31 @@ def test_cg_f32_ragged_length_and_unaligned_vectors():
     assert xx[0].item() == 0.0
 
 
+def test_cg_f64_ragged_length_and_unaligned_vectors():
+    """The FP64 vector updates run 16-byte passes (two doubles per lane) when
+    r, y, x and p are 16-byte aligned and finish an odd last entry one per
+    lane: n = 1001 covers the tail, and b and x one double past a 16-byte
+    boundary cover the one-per-lane fallback for x.  Both keep the CG
+    contract against the oracle (iterations within one, residual history
+    within 1e-10) and give the same x bit for bit."""
+    import ctypes
+    A = P.stencil(7, 7, 11, 13)
+    assert A.nrows % 2 == 1
+    b = P.uniform(4, A.nrows, -1, 1)
+    x, k, xo, ko = check_cg(A, b, 5000, 1e-10)
+    M = S.Matrix(A.rowptr, A.colidx, A.val, ncols=A.ncols)
+    bb = torch.zeros(A.nrows + 1, dtype=torch.float64, device=DEV)
+    xx = torch.zeros(A.ncols + 1, dtype=torch.float64, device=DEV)
+    bb[1:] = _dev(b)
+    assert bb[1:].data_ptr() % 16 == 8 and xx[1:].data_ptr() % 16 == 8
+    its = ctypes.c_int32()
+    S._check(S.lib().spmvx_cg(M.handle, None, S._dptr(bb[1:]), S._dptr(xx[1:]), 5000, 1e-10,
+                              ctypes.byref(its), None, None))
+    assert its.value == k
+    assert np.array_equal(xx[1:].cpu().numpy(), x)
+    assert xx[0].item() == 0.0
+
+
 # ---------------------------------------------------------------- sliced layout
 @pytest.mark.parametrize("name,A,rtol", [
     ("poisson7_24", P.stencil(7, 24, 24, 24), 1e-10),
