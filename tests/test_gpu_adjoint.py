@@ -126,8 +126,8 @@ def test_config_diffusion_1024_matches_oracle_adjoint():
     (the reference's CG for the forward and the adjoint solve, products and
     axpys over the host's threads, dot products in a fixed 64-chunk order):
     loss, grad_b = lambda and grad_theta = -(lambda_i - lambda_j)(x_i - x_j)
-    per edge.  The two CG runs round their dot products differently (8048 +
-    7469-7475 iterations on the GPU, 8065 + 7483 on the host), so they agree to
+    per edge.  The two CG runs round their dot products differently (~8050 +
+    ~7470 iterations on the GPU, 8065 + 7483 on the host), so they agree to
     1e-14 relative in x and lambda and 7e-12 in grad_theta, not bit for bit;
     the bounds hold a margin of ~1000."""
     import os
