@@ -105,9 +105,9 @@ def run_c1_section(torch, S, P, O, do_cpu, nthreads, n=512, napply=100, rtol=1e-
     out["gpu_ms_per_apply"] = 1e3 * tg / napply
     out["spmv_bytes"] = Bs
     xt = P.uniform(12, A.ncols, -1, 1)
-    oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)
-    b = O.spmv(oc, xt)
-    bd = torch.from_numpy(b).to(dev)
+    oc = O.CSR(A.nrows, A.ncols, A.rowptr, A.colidx, A.val)  # the checker and the CPU legs
+    bd = M.mult(torch.from_numpy(xt).to(dev))                 # b = A x_true on the GPU
+    b = bd.cpu().numpy()
     S.cg(M, bd, 2, rtol)
     torch.cuda.synchronize()
     t = time.perf_counter()
